@@ -1,0 +1,130 @@
+// C ABI over block sources: .tree files and trees built in host RAM.
+#include <cstring>
+#include <memory>
+
+#include "builder.h"
+#include "rt_internal.h"
+#include "tree_handle.h"
+
+using rayt::BlockSource;
+
+namespace {
+
+int FinishBuild(const rayt::NodeTree& t, int nodes_in_block, rt_tree** out) {
+  std::string err;
+  rayt::BuildReport rep;
+  std::unique_ptr<rayt::MemoryTree> mt = rayt::BuildOctree(t, nodes_in_block, &rep, &err);
+  if (!mt) return rayt::Fail(RT_ERR_INVALID, "build failed: " + err);
+  if (rep.allocated_groups != rep.deallocated_groups)
+    return rayt::Fail(RT_ERR_STATE, "builder leaked sibling groups");
+  rt_tree* h = new rt_tree();
+  h->mem = mt.get();
+  h->src = std::move(mt);
+  *out = h;
+  return RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_tree_open(const char* path, rt_tree** out) {
+  RT_REQUIRE(path && out, RT_ERR_INVALID, "null argument");
+  RT_TRY_BEGIN
+  std::unique_ptr<rayt::StoredOctreeLoader> l(new rayt::StoredOctreeLoader(path));
+  if (!l->ok()) return rayt::Fail(RT_ERR_IO, l->error());
+  rt_tree* h = new rt_tree();
+  h->src = std::move(l);
+  *out = h;
+  return RT_OK;
+  RT_TRY_END
+}
+
+int rt_tree_build_sphere(float cx, float cy, float cz, float radius, int level, int nodes_in_block, rt_tree** out) {
+  RT_REQUIRE(out, RT_ERR_INVALID, "null out");
+  RT_REQUIRE(level >= 0 && level <= 12, RT_ERR_RANGE, "sphere level must be in [0, 12]");
+  RT_TRY_BEGIN
+  rayt::NodeTree t = rayt::GenerateSphereTree(cx, cy, cz, radius, level);
+  return FinishBuild(t, nodes_in_block, out);
+  RT_TRY_END
+}
+
+int rt_tree_build_terrain(const rt_terrain_params* p, int nodes_in_block, rt_tree** out) {
+  RT_REQUIRE(p && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(p->depth >= 1 && p->depth <= 15, RT_ERR_RANGE, "terrain depth must be in [1, 15]");
+  RT_REQUIRE(p->octaves >= 1 && p->octaves <= 16, RT_ERR_RANGE, "octaves must be in [1, 16]");
+  RT_REQUIRE(p->spheres >= 0 && p->boxes >= 0 && p->spheres + p->boxes <= 127, RT_ERR_RANGE,
+             "at most 127 spheres + boxes");
+  RT_TRY_BEGIN
+  rayt::TerrainParams tp;
+  tp.depth = p->depth;
+  tp.seed = p->seed;
+  tp.octaves = p->octaves;
+  tp.height_lo = p->height_lo;
+  tp.height_hi = p->height_hi;
+  tp.spheres = p->spheres;
+  tp.sphere_rmin = p->sphere_rmin;
+  tp.sphere_rmax = p->sphere_rmax;
+  tp.boxes = p->boxes;
+  tp.box_emin = p->box_emin;
+  tp.box_emax = p->box_emax;
+  tp.threads = p->threads;
+  rayt::NodeTree t = rayt::GenerateTerrainTree(tp);
+  return FinishBuild(t, nodes_in_block, out);
+  RT_TRY_END
+}
+
+int rt_tree_write(const rt_tree* tree, const char* path) {
+  RT_REQUIRE(tree && path, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(tree->mem, RT_ERR_STATE, "only trees built in memory can be written");
+  std::string err;
+  if (!tree->mem->WriteFile(path, &err)) return rayt::Fail(RT_ERR_IO, err);
+  return RT_OK;
+}
+
+int rt_tree_info_get(const rt_tree* tree, rt_tree_info* out) {
+  RT_REQUIRE(tree && out, RT_ERR_INVALID, "null argument");
+  const auto& h = tree->src->header();
+  out->nodes_in_block = h.nodes_in_block;
+  out->blocks_count = h.blocks_count;
+  out->root_block_index = h.root_block_index;
+  out->channels_count = uint32_t(h.channels.size());
+  out->bytes_in_block_content = tree->src->BytesInBlockContent();
+  out->source_nodes = tree->mem ? tree->mem->total_nodes : 0;
+  return RT_OK;
+}
+
+int rt_tree_channel(const rt_tree* tree, int index, int* bytes_in_node, char* name, int name_cap) {
+  RT_REQUIRE(tree, RT_ERR_INVALID, "null tree");
+  const auto& h = tree->src->header();
+  RT_REQUIRE(index >= 0 && size_t(index) < h.channels.size(), RT_ERR_RANGE, "channel index out of range");
+  if (bytes_in_node) *bytes_in_node = h.channels[index].bytes_in_node;
+  if (name && name_cap > 0) {
+    size_t n = std::min(size_t(name_cap - 1), h.channels[index].name.size());
+    memcpy(name, h.channels[index].name.data(), n);
+    name[n] = 0;
+  }
+  return RT_OK;
+}
+
+int rt_tree_load_block(rt_tree* tree, uint32_t index, uint8_t* header76, uint8_t* payload, uint64_t payload_cap) {
+  RT_REQUIRE(tree, RT_ERR_INVALID, "null tree");
+  RT_REQUIRE(index < tree->src->header().blocks_count, RT_ERR_RANGE, "block index out of range");
+  RT_TRY_BEGIN
+  rayt::StoredOctreeBlock b;
+  if (!tree->src->LoadBlock(index, &b)) return rayt::Fail(RT_ERR_IO, "failed to load block");
+  if (header76) rayt::EncodeBlockHeader(b.header, header76);
+  if (payload) {
+    RT_REQUIRE(payload_cap >= b.data.size(), RT_ERR_RANGE, "payload buffer too small");
+    memcpy(payload, b.data.data(), b.data.size());
+  }
+  return RT_OK;
+  RT_TRY_END
+}
+
+int rt_tree_destroy(rt_tree* tree) {
+  delete tree;
+  return RT_OK;
+}
+
+}  // extern "C"

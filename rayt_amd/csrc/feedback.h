@@ -1,0 +1,74 @@
+// Feedback extraction (RayTracingFeedbackExtractor,
+// GPU_renderer/ray-tracing-feedback-extractor.cpp:9-205) and the GPU
+// sort/scan primitives (GPU algorithms/gpu-sort.cpp, gpu-scan.cpp).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rt_internal.h"
+
+namespace rayt {
+
+int SortI32(hipStream_t s, int* data, int n, int* temp);
+int ScanI32(hipStream_t s, int* data, int n);
+
+class FeedbackExtractor {
+ public:
+  FeedbackExtractor(rt_ctx* ctx, int count);
+  ~FeedbackExtractor();
+  FeedbackExtractor(const FeedbackExtractor&) = delete;
+  FeedbackExtractor& operator=(const FeedbackExtractor&) = delete;
+
+  int count() const { return count_; }
+  int hits_to_extract() const { return hits_to_extract_; }
+  void set_hits_to_extract(int v) { hits_to_extract_ = v; }
+  void set_seed(uint64_t s) { rng_ = s; }
+  uint64_t seed_state() const { return rng_; }
+
+  // `codes` is the device faults_and_hits buffer (count_ ints). With
+  // block_level the hit codes are already ((slot+1)<<1)|dup.
+  int Process(int* codes, int nodes_in_block, bool block_level);
+
+  const std::vector<int>& fault_blocks() const { return faults_; }
+  const std::vector<int>& hit_blocks() const { return hits_; }
+  const std::vector<int>& duplicate_hit_blocks() const { return dup_hits_; }
+  int unique_count() const { return unique_count_; }
+  int expected_faults() const { return expected_faults_; }
+
+ private:
+  rt_ctx* ctx_;
+  int count_;
+  int hits_to_extract_ = 4000;
+  int expected_faults_ = 1 << 16;
+  uint64_t rng_ = 0;
+  int unique_count_ = 0, fault_count_ = 0;
+  std::vector<int> faults_, hits_, dup_hits_;
+  uint32_t* fault_bits_ = nullptr;
+  uint32_t* hit_bits_ = nullptr;
+  void* state_ = nullptr;
+  void* host_state_ = nullptr;
+  int* unique_ = nullptr;
+  int* temp_ = nullptr;
+  int* index_ = nullptr;
+  std::vector<int> host_sorted_, window_;
+
+  void AddHit(int encoded) {
+    if (encoded & 1)
+      dup_hits_.push_back((encoded >> 1) - 1);
+    else
+      hits_.push_back((encoded >> 1) - 1);
+  }
+  uint32_t NextRandom();
+  int UniqueBySort(int* codes);
+  int ReadLists();
+};
+
+}  // namespace rayt
+
+struct rt_feedback {
+  std::unique_ptr<rayt::FeedbackExtractor> fx;
+};

@@ -1,0 +1,630 @@
+// Feedback pipeline on gfx950: the reference's GPU algorithms (GPUSort,
+// GPUScan, NodeToBlock / DifferenceFlag / Compress) and the extraction of the
+// fault / hit block lists (RayTracingFeedbackExtractor).
+//
+// The extractor does not sort. Every feedback code lives in a small known
+// domain (fault blocks < 2^21, hit codes < 2 * pool blocks + 2), so the set of
+// distinct codes is marked in two bitmaps and a single workgroup walks the
+// bitmaps in code order - fault bitmap from the highest block down, then the
+// hit bitmap upwards, then NO_HIT - emitting exactly the array that
+// sort + DifferenceFlag + Scan + Compress produce. Codes outside the domain
+// fall back to the radix-sort path.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "feedback.h"
+#include "rt_internal.h"
+
+namespace rayt {
+
+namespace {
+
+constexpr int kNoHit = 0x7fffffff;
+constexpr uint32_t kFaultDomain = 1u << 21;  // fault links carry 21-bit block indices
+constexpr uint32_t kHitDomain = 1u << 23;    // hit codes handled by the bitmap path
+
+// ---------------------------------------------------------------- wave scans
+__device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread (BLOCK a multiple of 64).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* lds_waves, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t inc = WaveInclusiveScan(v);
+  if (lane == 63) lds_waves[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      const uint32_t t = lds_waves[w];
+      lds_waves[w] = run;
+      run += t;
+    }
+    lds_waves[BLOCK / 64] = run;
+  }
+  __syncthreads();
+  const uint32_t res = lds_waves[wave] + inc - v;
+  total = lds_waves[BLOCK / 64];
+  __syncthreads();
+  return res;
+}
+
+// ---------------------------------------------------- feedback_compression.cl
+__global__ void NodeToBlockKernel(int n, int block_size, int* __restrict__ data) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int d = data[i];
+  if (d > 0 && d != kNoHit) data[i] = ((((d >> 1) - 1) / block_size + 1) << 1) | (d & 1);
+}
+
+__global__ void DifferenceFlagKernel(int n, const int* __restrict__ in, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (i != 0 && in[i - 1] != in[i]) ? 1 : 0;
+}
+
+__global__ void CompressKernel(int n, const int* __restrict__ index, const int* __restrict__ in, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i == 0)
+    out[0] = in[0];
+  else if (in[i - 1] != in[i])
+    out[index[i]] = in[i];
+}
+
+// ------------------------------------------------------------- radix sort
+// LSD radix sort of int32 keys (sign bit flipped), 8-bit digits, 4 passes.
+// Per tile of 256 x 8 keys: a digit histogram (upsweep), one scan over all
+// (digit, tile) counts, and a stable scatter that sorts the tile in LDS with
+// eight 1-bit block-wide splits before writing runs out.
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;
+
+__device__ __forceinline__ uint32_t SortKey(int v) { return uint32_t(v) ^ 0x80000000u; }
+
+__global__ __launch_bounds__(kSortBlock) void RadixUpsweepKernel(const int* __restrict__ in, int n, int shift,
+                                                                 uint32_t* __restrict__ hist, int tiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const int i = base + k * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(SortKey(in[i]) >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `n` uint32 in place with one workgroup of 1024 lanes.
+__global__ __launch_bounds__(1024) void ScanSingleBlockKernel(uint32_t* __restrict__ data, int n) {
+  __shared__ uint32_t waves[1024 / 64 + 1];
+  const int per = (n + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(n, b + per);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += data[i];
+  uint32_t total;
+  uint32_t run = BlockExclusiveScan<1024>(s, waves, total);
+  for (int i = b; i < e; ++i) {
+    const uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortBlock) void RadixScatterKernel(const int* __restrict__ in, int* __restrict__ out,
+                                                                 int n, int shift, const uint32_t* __restrict__ hist,
+                                                                 int tiles) {
+  __shared__ uint32_t keys[kSortTile];
+  __shared__ uint32_t tmp[kSortTile];
+  __shared__ uint32_t waves[kSortBlock / 64 + 1];
+  __shared__ uint32_t digit_start[256];
+  __shared__ uint32_t digit_global[256];
+  const int base = blockIdx.x * kSortTile;
+  const int valid = min(kSortTile, n - base);
+  // Blocked arrangement: thread t owns keys [t*8, t*8+8) of the tile, in tile order.
+  uint32_t k[kSortItems];
+  for (int j = 0; j < kSortItems; ++j) {
+    const int i = threadIdx.x * kSortItems + j;
+    k[j] = i < valid ? SortKey(in[base + i]) : 0xffffffffu;  // padding sorts last and is never written
+  }
+  // Stable sort of the tile by the current digit: 8 one-bit splits.
+  for (int bit = 0; bit < 8; ++bit) {
+    uint32_t zeros = 0;
+    for (int j = 0; j < kSortItems; ++j) zeros += ((k[j] >> (shift + bit)) & 1u) ? 0u : 1u;
+    uint32_t total_zeros;
+    const uint32_t zero_before = BlockExclusiveScan<kSortBlock>(zeros, waves, total_zeros);
+    uint32_t zpos = zero_before;
+    uint32_t opos = total_zeros + threadIdx.x * kSortItems - zero_before;
+    for (int j = 0; j < kSortItems; ++j) {
+      if ((k[j] >> (shift + bit)) & 1u)
+        tmp[opos++] = k[j];
+      else
+        tmp[zpos++] = k[j];
+    }
+    __syncthreads();
+    for (int j = 0; j < kSortItems; ++j) k[j] = tmp[threadIdx.x * kSortItems + j];
+    __syncthreads();
+  }
+  for (int j = 0; j < kSortItems; ++j) keys[threadIdx.x * kSortItems + j] = k[j];
+  digit_global[threadIdx.x] = hist[threadIdx.x * tiles + blockIdx.x];
+  digit_start[threadIdx.x] = 0xffffffffu;
+  __syncthreads();
+  // First position of each digit in the sorted tile.
+  for (int j = 0; j < kSortItems; ++j) {
+    const int i = threadIdx.x * kSortItems + j;
+    if (i < valid) {
+      const uint32_t d = (keys[i] >> shift) & 255u;
+      if (i == 0 || ((keys[i - 1] >> shift) & 255u) != d) digit_start[d] = uint32_t(i);
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < kSortItems; ++j) {
+    const int i = j * kSortBlock + threadIdx.x;  // striped for coalesced runs
+    if (i < valid) {
+      const uint32_t key = keys[i];
+      const uint32_t d = (key >> shift) & 255u;
+      out[digit_global[d] + (uint32_t(i) - digit_start[d])] = int(key ^ 0x80000000u);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__global__ __launch_bounds__(kScanBlock) void ScanReduceKernel(const int* __restrict__ in, int n,
+                                                               uint32_t* __restrict__ sums) {
+  __shared__ uint32_t waves[kScanBlock / 64 + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t s = 0;
+  for (int j = 0; j < kScanItems; ++j)
+    if (base + j < n) s += uint32_t(in[base + j]);
+  uint32_t total;
+  BlockExclusiveScan<kScanBlock>(s, waves, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void ScanDownsweepKernel(const int* __restrict__ in, int* __restrict__ out,
+                                                                  int n, const uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t waves[kScanBlock / 64 + 1];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = base + j < n ? uint32_t(in[base + j]) : 0u;
+    s += v[j];
+  }
+  uint32_t total;
+  uint32_t run = BlockExclusiveScan<kScanBlock>(s, waves, total) + offsets[blockIdx.x];
+  for (int j = 0; j < kScanItems; ++j) {
+    run += v[j];
+    if (base + j < n) out[base + j] = int(run);
+  }
+}
+
+// ------------------------------------------------------ bitmap extraction
+struct MarkState {
+  int32_t max_fault;     // highest fault block seen (-1: none)
+  int32_t max_hit;       // highest hit code seen (-1: none)
+  int32_t no_hit;        // NO_HIT present
+  int32_t overflow;      // a code outside the bitmap domain
+  int32_t unique_count;  // outputs of the walk
+  int32_t fault_count;
+  int32_t pad[2];
+};
+
+__device__ __forceinline__ int WaveMax(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
+                                                       uint32_t* __restrict__ fault_bits, uint32_t* __restrict__ hit_bits,
+                                                       MarkState* __restrict__ st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int code = i < n ? codes[i] : kNoHit;
+  if (block_size > 0 && code > 0 && code != kNoHit) code = ((((code >> 1) - 1) / block_size + 1) << 1) | (code & 1);
+  // Skip lanes repeating the previous lane's code (coherent images have long runs).
+  const int prev = __shfl_up(code, 1, 64);
+  const bool dup = ((threadIdx.x & 63) != 0) && prev == code;
+  int mf = -1, mh = -1;
+  bool nohit = false, over = false;
+  if (i < n && !dup) {
+    if (code == kNoHit) {
+      nohit = true;
+    } else if (code < 0) {
+      const uint32_t f = uint32_t(-(code + 1));
+      if (f >= kFaultDomain) {
+        over = true;
+      } else {
+        const uint32_t bit = 1u << (f & 31u);
+        if (!(fault_bits[f >> 5] & bit)) atomicOr(&fault_bits[f >> 5], bit);
+        mf = int(f);
+      }
+    } else {
+      const uint32_t h = uint32_t(code);
+      if (h >= kHitDomain) {
+        over = true;
+      } else {
+        const uint32_t bit = 1u << (h & 31u);
+        if (!(hit_bits[h >> 5] & bit)) atomicOr(&hit_bits[h >> 5], bit);
+        mh = int(h);
+      }
+    }
+  }
+  mf = WaveMax(mf);
+  mh = WaveMax(mh);
+  const bool any_nohit = __any(nohit), any_over = __any(over);
+  if ((threadIdx.x & 63) == 0) {
+    if (mf >= 0) atomicMax(&st->max_fault, mf);
+    if (mh >= 0) atomicMax(&st->max_hit, mh);
+    if (any_nohit) atomicOr(&st->no_hit, 1);
+    if (any_over) atomicOr(&st->overflow, 1);
+  }
+}
+
+// One workgroup walks the bitmaps in ascending code order and writes the
+// distinct codes, then clears what it read and resets the state.
+__global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
+                                                          uint32_t* __restrict__ hit_bits, MarkState* __restrict__ st,
+                                                          int* __restrict__ unique_out) {
+  __shared__ uint32_t waves[1024 / 64 + 1];
+  const int wf = st->max_fault >= 0 ? (st->max_fault >> 5) + 1 : 0;
+  const int wh = st->max_hit >= 0 ? (st->max_hit >> 5) + 1 : 0;
+  const int L = wf + wh;
+  const int per = (L + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(L, b + per);
+  auto word = [&](int j) -> uint32_t { return j < wf ? fault_bits[wf - 1 - j] : hit_bits[j - wf]; };
+  uint32_t cnt = 0;
+  for (int j = b; j < e; ++j) cnt += __popc(word(j));
+  uint32_t total;
+  uint32_t pos = BlockExclusiveScan<1024>(cnt, waves, total);
+  uint32_t faults_total;
+  {
+    uint32_t fc = 0;
+    for (int j = b; j < min(e, wf); ++j) fc += __popc(word(j));
+    BlockExclusiveScan<1024>(fc, waves, faults_total);
+  }
+  for (int j = b; j < e; ++j) {
+    uint32_t w = word(j);
+    if (j < wf) {
+      const int fw = wf - 1 - j;
+      while (w) {  // descending block index = ascending code
+        const int bit = 31 - __clz(w);
+        w &= ~(1u << bit);
+        unique_out[pos++] = -(fw * 32 + bit) - 1;
+      }
+      fault_bits[fw] = 0u;
+    } else {
+      const int hw = j - wf;
+      while (w) {
+        const int bit = __ffs(w) - 1;
+        w &= w - 1;
+        unique_out[pos++] = hw * 32 + bit;
+      }
+      hit_bits[hw] = 0u;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int u = int(total);
+    if (st->no_hit) unique_out[u++] = kNoHit;
+    st->unique_count = u;
+    st->fault_count = int(faults_total);
+    st->max_fault = -1;
+    st->max_hit = -1;
+    st->no_hit = 0;
+  }
+}
+
+int Grid(int64_t n, int block) { return int((n + block - 1) / block); }
+
+}  // namespace
+
+// ------------------------------------------------------------ host helpers
+int SortI32(hipStream_t s, int* data, int n, int* temp) {
+  if (n <= 1) return RT_OK;
+  const int tiles = Grid(n, kSortTile);
+  uint32_t* hist = nullptr;
+  RT_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&hist), size_t(tiles) * 256 * 4, s));
+  int* a = data;
+  int* b = temp;
+  for (int shift = 0; shift < 32; shift += 8) {
+    RadixUpsweepKernel<<<dim3(tiles), dim3(kSortBlock), 0, s>>>(a, n, shift, hist, tiles);
+    ScanSingleBlockKernel<<<dim3(1), dim3(1024), 0, s>>>(hist, tiles * 256);
+    RadixScatterKernel<<<dim3(tiles), dim3(kSortBlock), 0, s>>>(a, b, n, shift, hist, tiles);
+    std::swap(a, b);
+  }
+  // four passes: the result is back in `data`
+  hipError_t e = hipGetLastError();
+  (void)hipFreeAsync(hist, s);
+  if (e != hipSuccess) return Fail(RT_ERR_HIP, hipGetErrorString(e));
+  return RT_OK;
+}
+
+int ScanI32(hipStream_t s, int* data, int n) {
+  if (n <= 0) return RT_OK;
+  const int tiles = Grid(n, kScanTile);
+  uint32_t* sums = nullptr;
+  RT_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&sums), size_t(tiles) * 4, s));
+  ScanReduceKernel<<<dim3(tiles), dim3(kScanBlock), 0, s>>>(data, n, sums);
+  ScanSingleBlockKernel<<<dim3(1), dim3(1024), 0, s>>>(sums, tiles);
+  ScanDownsweepKernel<<<dim3(tiles), dim3(kScanBlock), 0, s>>>(data, data, n, sums);
+  hipError_t e = hipGetLastError();
+  (void)hipFreeAsync(sums, s);
+  if (e != hipSuccess) return Fail(RT_ERR_HIP, hipGetErrorString(e));
+  return RT_OK;
+}
+
+// --------------------------------------------------------- FeedbackExtractor
+FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(ctx), count_(count) {
+  auto alloc = [&](void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+  };
+  alloc(reinterpret_cast<void**>(&fault_bits_), kFaultDomain / 8);
+  alloc(reinterpret_cast<void**>(&hit_bits_), kHitDomain / 8);
+  alloc(reinterpret_cast<void**>(&state_), sizeof(MarkState));
+  alloc(reinterpret_cast<void**>(&unique_), size_t(std::max(count, 1) + 1) * 4);
+  alloc(reinterpret_cast<void**>(&temp_), size_t(std::max(count, 1)) * 4);
+  alloc(reinterpret_cast<void**>(&index_), size_t(std::max(count, 1)) * 4);
+  (void)hipMemsetAsync(fault_bits_, 0, kFaultDomain / 8, ctx_->stream);
+  (void)hipMemsetAsync(hit_bits_, 0, kHitDomain / 8, ctx_->stream);
+  MarkState init{-1, -1, 0, 0, 0, 0, {0, 0}};
+  (void)hipMemcpyAsync(state_, &init, sizeof(init), hipMemcpyHostToDevice, ctx_->stream);
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_state_), sizeof(MarkState), hipHostMallocDefault);
+  if (e != hipSuccess) throw std::runtime_error("hipHostMalloc");
+  host_sorted_.resize(size_t(std::max(count, 1)) + 1);
+  (void)hipStreamSynchronize(ctx_->stream);
+}
+
+FeedbackExtractor::~FeedbackExtractor() {
+  (void)hipStreamSynchronize(ctx_->stream);
+  (void)hipFree(fault_bits_);
+  (void)hipFree(hit_bits_);
+  (void)hipFree(state_);
+  (void)hipFree(unique_);
+  (void)hipFree(temp_);
+  (void)hipFree(index_);
+  (void)hipHostFree(host_state_);
+}
+
+uint32_t FeedbackExtractor::NextRandom() {
+  uint64_t z = (rng_ += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return uint32_t(z >> 33);
+}
+
+// Sorted distinct codes via the reference's own sequence of steps (sort,
+// difference flags, scan, compress), used when codes leave the bitmap domain.
+int FeedbackExtractor::UniqueBySort(int* codes) {
+  hipStream_t s = ctx_->stream;
+  int rc = SortI32(s, codes, count_, temp_);
+  if (rc) return rc;
+  DifferenceFlagKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, codes, index_);
+  rc = ScanI32(s, index_, count_);
+  if (rc) return rc;
+  CompressKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, index_, codes, unique_);
+  RT_HIP_CHECK(hipGetLastError());
+  int last = 0;
+  RT_HIP_CHECK(hipMemcpyAsync(&last, index_ + (count_ - 1), 4, hipMemcpyDeviceToHost, s));
+  RT_HIP_CHECK(hipStreamSynchronize(s));
+  unique_count_ = last + 1;
+  // fault count = number of negative distinct codes
+  std::vector<int> tmp(unique_count_);
+  RT_HIP_CHECK(hipMemcpy(tmp.data(), unique_, size_t(unique_count_) * 4, hipMemcpyDeviceToHost));
+  fault_count_ = int(std::lower_bound(tmp.begin(), tmp.end(), 0) - tmp.begin());
+  return RT_OK;
+}
+
+int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level) {
+  hipStream_t s = ctx_->stream;
+  if (count_ <= 0) {
+    faults_.clear();
+    hits_.clear();
+    dup_hits_.clear();
+    unique_count_ = 0;
+    return RT_OK;
+  }
+  const int bs = block_level ? 0 : nodes_in_block;
+  MarkState* st = static_cast<MarkState*>(state_);
+  MarkCodesKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_, st);
+  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_);
+  RT_HIP_CHECK(hipGetLastError());
+  RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));
+  RT_HIP_CHECK(hipStreamSynchronize(s));
+  const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
+  if (hs.overflow) {
+    int zero = 0;
+    RT_HIP_CHECK(hipMemcpyAsync(&reinterpret_cast<MarkState*>(state_)->overflow, &zero, 4, hipMemcpyHostToDevice, s));
+    if (!block_level) NodeToBlockKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, nodes_in_block, codes);
+    int rc = UniqueBySort(codes);
+    if (rc) return rc;
+  } else {
+    unique_count_ = hs.unique_count;
+    fault_count_ = hs.fault_count;
+  }
+  return ReadLists();
+}
+
+// ProcessUnique + AdjustExpectedFaults (ray-tracing-feedback-extractor.cpp:108-185):
+// the reads of growing strides until the first non-fault are replayed from the
+// known fault count, then one window of hits is read from a random start.
+int FeedbackExtractor::ReadLists() {
+  hipStream_t s = ctx_->stream;
+  faults_.clear();
+  hits_.clear();
+  dup_hits_.clear();
+  const int U = unique_count_, F = fault_count_;
+  int stride = expected_faults_, size = 0;
+  while (size < U && (size == 0 || size <= F)) {  // sorted[size-1] < 0  <=>  size-1 < F
+    stride = std::min(stride, U - size);
+    size += stride;
+    stride *= 2;
+  }
+  int window_start = 0, window_len = 0, hits_length = 0;
+  if (size < U) {
+    hits_length = U - size;
+    window_start = int(NextRandom() % uint32_t(hits_length));
+    window_len = std::min(hits_to_extract_, hits_length);
+  }
+  if (size > 0) RT_HIP_CHECK(hipMemcpyAsync(host_sorted_.data(), unique_, size_t(size) * 4, hipMemcpyDeviceToHost, s));
+  std::vector<std::pair<int, int>> segs;
+  if (window_len > 0) {
+    if (window_start + window_len <= hits_length) {
+      segs.push_back({window_start, window_len});
+    } else {
+      segs.push_back({0, window_start + window_len - hits_length});
+      segs.push_back({window_start, hits_length - window_start});
+    }
+  }
+  window_.resize(size_t(window_len));
+  int w = 0;
+  for (auto& sg : segs) {
+    RT_HIP_CHECK(hipMemcpyAsync(window_.data() + w, unique_ + size + sg.first, size_t(sg.second) * 4,
+                                hipMemcpyDeviceToHost, s));
+    w += sg.second;
+  }
+  RT_HIP_CHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < size; ++i) {
+    const int v = host_sorted_[i];
+    if (v < 0)
+      faults_.push_back(-v - 1);
+    else if (v != kNoHit)
+      AddHit(v);
+  }
+  for (int i = 0; i < window_len; ++i)
+    if (window_[i] != kNoHit) AddHit(window_[i]);
+  if (expected_faults_ > 1 && int(faults_.size()) < expected_faults_)
+    expected_faults_ /= 2;
+  else if (int(faults_.size()) > expected_faults_)
+    expected_faults_ *= 2;
+  return RT_OK;
+}
+
+}  // namespace rayt
+
+// -------------------------------------------------------------------- C ABI
+extern "C" {
+
+int rt_node_to_block(rt_ctx* ctx, int n, int block_size, rt_buf* data) {
+  RT_REQUIRE(ctx && data, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= data->size, RT_ERR_RANGE, "n exceeds buffer");
+  RT_REQUIRE(block_size > 0, RT_ERR_INVALID, "block_size must be positive");
+  if (!n) return RT_OK;
+  rayt::NodeToBlockKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(n, block_size,
+                                                                                   static_cast<int*>(data->ptr));
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_difference_flag(rt_ctx* ctx, int n, const rt_buf* in, rt_buf* out) {
+  RT_REQUIRE(ctx && in && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= in->size && uint64_t(n) * 4 <= out->size, RT_ERR_RANGE, "n exceeds buffer");
+  if (!n) return RT_OK;
+  rayt::DifferenceFlagKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(
+      n, static_cast<const int*>(in->ptr), static_cast<int*>(out->ptr));
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_compress(rt_ctx* ctx, int n, const rt_buf* index, const rt_buf* in, rt_buf* out) {
+  RT_REQUIRE(ctx && index && in && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= in->size && uint64_t(n) * 4 <= index->size && uint64_t(n) * 4 <= out->size,
+             RT_ERR_RANGE, "n exceeds buffer");
+  if (!n) return RT_OK;
+  rayt::CompressKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(
+      n, static_cast<const int*>(index->ptr), static_cast<const int*>(in->ptr), static_cast<int*>(out->ptr));
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_sort_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp) {
+  RT_REQUIRE(ctx && data && temp, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= data->size && uint64_t(n) * 4 <= temp->size, RT_ERR_RANGE,
+             "n exceeds buffer");
+  return rayt::SortI32(ctx->stream, static_cast<int*>(data->ptr), n, static_cast<int*>(temp->ptr));
+}
+
+int rt_scan_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp) {
+  RT_REQUIRE(ctx && data, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= data->size, RT_ERR_RANGE, "n exceeds buffer");
+  (void)temp;
+  return rayt::ScanI32(ctx->stream, static_cast<int*>(data->ptr), n);
+}
+
+int rt_feedback_create(rt_ctx* ctx, int count, rt_feedback** out) {
+  RT_REQUIRE(ctx && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(count >= 0, RT_ERR_RANGE, "negative count");
+  RT_TRY_BEGIN
+  rt_feedback* f = new rt_feedback();
+  try {
+    f->fx.reset(new rayt::FeedbackExtractor(ctx, count));
+  } catch (...) {
+    delete f;
+    throw;
+  }
+  *out = f;
+  return RT_OK;
+  RT_TRY_END
+}
+
+int rt_feedback_destroy(rt_feedback* fb) {
+  delete fb;
+  return RT_OK;
+}
+
+int rt_feedback_set_hits_to_extract(rt_feedback* fb, int value) {
+  RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  RT_REQUIRE(value >= 0, RT_ERR_RANGE, "negative hits_to_extract");
+  fb->fx->set_hits_to_extract(value);
+  return RT_OK;
+}
+
+int rt_feedback_set_seed(rt_feedback* fb, uint64_t seed) {
+  RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  fb->fx->set_seed(seed);
+  return RT_OK;
+}
+
+int rt_feedback_process(rt_feedback* fb, rt_buf* codes, int nodes_in_block, int block_level) {
+  RT_REQUIRE(fb && codes, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(uint64_t(fb->fx->count()) * 4 <= codes->size, RT_ERR_RANGE, "codes buffer smaller than count");
+  RT_REQUIRE(block_level || nodes_in_block > 0, RT_ERR_INVALID, "nodes_in_block must be positive");
+  RT_TRY_BEGIN
+  return fb->fx->Process(static_cast<int*>(codes->ptr), nodes_in_block, block_level != 0);
+  RT_TRY_END
+}
+
+int rt_feedback_list(const rt_feedback* fb, int which, int* out, int cap) {
+  RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  const std::vector<int>* v = which == 0 ? &fb->fx->fault_blocks() : which == 1 ? &fb->fx->hit_blocks()
+                                                                               : &fb->fx->duplicate_hit_blocks();
+  RT_REQUIRE(which >= 0 && which <= 2, RT_ERR_RANGE, "which must be 0, 1 or 2");
+  for (int i = 0; i < cap && i < int(v->size()); ++i) out[i] = (*v)[i];
+  return int(v->size());
+}
+
+int rt_feedback_stats(const rt_feedback* fb, int* unique_count, int* expected_faults) {
+  RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  if (unique_count) *unique_count = fb->fx->unique_count();
+  if (expected_faults) *expected_faults = fb->fx->expected_faults();
+  return RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Internal definitions shared by the C-ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <string>
+
+#include "../../include/rayt_hip.h"
+
+struct rt_ctx {
+  int device = 0;
+  int compute_units = 0;
+  hipStream_t stream = nullptr;
+  int plain_lds_levels = 16;  // LDS stack levels for the plain kernel
+  int plain_tile = 0;         // 0: 16x16 workgroup of 8x8 wave tiles
+};
+
+struct rt_buf {
+  rt_ctx* ctx = nullptr;
+  void* ptr = nullptr;
+  uint64_t size = 0;
+  bool owned = true;
+};
+
+struct rt_evt {
+  hipEvent_t ev = nullptr;
+  std::atomic<int> refs{1};
+};
+
+namespace rayt {
+
+void SetError(const std::string& msg);
+int Fail(int code, const std::string& msg);
+const char* LastError();
+
+}  // namespace rayt
+
+#define RT_HIP_CHECK(call)                                                                    \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return rayt::Fail(RT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RT_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) return rayt::Fail((code), (msg)); \
+  } while (0)
+
+#define RT_TRY_BEGIN try {
+#define RT_TRY_END                                              \
+  }                                                             \
+  catch (const std::bad_alloc&) {                               \
+    return rayt::Fail(RT_ERR_NOMEM, "host allocation failed");  \
+  }                                                             \
+  catch (const std::exception& e) {                             \
+    return rayt::Fail(RT_ERR_STATE, e.what());                  \
+  }

@@ -1,0 +1,308 @@
+// Device-side sparse-voxel-octree traversal for gfx950.
+//
+// Semantics follow the reference kernels operation for operation so that hit
+// node ids are bit-identical to the CPU restatement:
+//   CastRay          GPU_renderer/kernels/raycast.cl:33-256
+//   InitTracingFrame GPU_renderer/kernels/init_tracing_frame.cl:3-45
+//   ProcessHit       GPU_renderer/kernels/shading.cl:4-33
+//   FinishTracingFrame GPU_renderer/kernels/finish_tracing_frame.cl:3-24
+// FP32 rules (fixed here because OpenCL leaves them to the implementation):
+// no contraction (built with -ffp-contract=off), correctly rounded / and sqrt,
+// dot = ((x*a + y*b) + z*c) + w*d, normalize(v) = v / sqrt(dot(v,v)),
+// distance(a,b) = sqrt(dot(a-b,a-b)), fmax/fmin/max/min as the OpenCL
+// definitions "y if x<y else x" / "y if y<x else x".
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtdev {
+
+constexpr int kSMax = 23;                  // raycast.cl:10
+constexpr float kEpsilon = 0.00000011920928955078125f;  // exp2(-S_MAX), raycast.cl:11
+constexpr int kIterLimit = 300;            // raycast.cl:95
+constexpr int kNoHit = 0x7fffffff;         // common.cl:14
+constexpr int kStackLevels = kSMax - 1;    // levels 0..21 (scale 22..1)
+
+__device__ __forceinline__ float cl_fmax(float x, float y) {
+  if (x != x) return y;
+  return (x < y) ? y : x;
+}
+__device__ __forceinline__ float cl_fmin(float x, float y) {
+  if (x != x) return y;
+  return (y < x) ? y : x;
+}
+__device__ __forceinline__ float cl_max(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float cl_min(float x, float y) { return (y < x) ? y : x; }
+
+struct CastResult {
+  float hit_t;
+  int hit_node;
+  int fault_block;
+  int hit_duplicate;
+  int error_code;
+  int node_depth;
+};
+
+// Traversal stack indexed by level = (S_MAX-1) - scale. Levels below LDS_LEVELS
+// live in LDS (one column per thread, lanes contiguous so a wave touches
+// consecutive 8-byte words); deeper levels live in a small private array that
+// is only touched for trees deeper than LDS_LEVELS+1.
+template <int LDS_LEVELS, int BLOCK>
+struct SplitStack {
+  uint2* lds;  // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
+  uint2 spill[(kStackLevels - LDS_LEVELS) > 0 ? (kStackLevels - LDS_LEVELS) : 1];
+  __device__ __forceinline__ void put(int scale, uint2 v) {
+    int level = (kSMax - 1) - scale;
+    if (LDS_LEVELS >= kStackLevels || level < LDS_LEVELS)
+      lds[level * BLOCK] = v;
+    else
+      spill[level - LDS_LEVELS] = v;
+  }
+  __device__ __forceinline__ uint2 get(int scale) const {
+    int level = (kSMax - 1) - scale;
+    if (LDS_LEVELS >= kStackLevels || level < LDS_LEVELS) return lds[level * BLOCK];
+    return spill[level - LDS_LEVELS];
+  }
+};
+
+// ESVO traversal over the cache link format. `fetches` counts executed
+// node-link and far-pointer loads (the F of the roofline accounting).
+template <class Stack, bool COUNT>
+__device__ __forceinline__ void CastRay(const uint32_t* __restrict__ node_links,
+                                        const uint32_t* __restrict__ far_pointers, uint32_t root_node,
+                                        float px, float py, float pz, float dx, float dy, float dz,
+                                        float ray_size_coef, float ray_size_bias, Stack& stack,
+                                        CastResult& res, uint32_t& fetches) {
+  px += 1.0f;
+  py += 1.0f;
+  pz += 1.0f;
+  res.fault_block = -1;
+  res.error_code = 0;
+  res.hit_duplicate = 0;
+
+  if (fabsf(dx) < kEpsilon) dx = copysignf(kEpsilon, dx);
+  if (fabsf(dy) < kEpsilon) dy = copysignf(kEpsilon, dy);
+  if (fabsf(dz) < kEpsilon) dz = copysignf(kEpsilon, dz);
+
+  const float tx_coef = 1.0f / -fabsf(dx);
+  const float ty_coef = 1.0f / -fabsf(dy);
+  const float tz_coef = 1.0f / -fabsf(dz);
+  float tx_bias = tx_coef * px;
+  float ty_bias = ty_coef * py;
+  float tz_bias = tz_coef * pz;
+
+  uint32_t octant_mask = 0;
+  if (dx > 0.0f) octant_mask ^= 1u, tx_bias = 3.0f * tx_coef - tx_bias;
+  if (dy > 0.0f) octant_mask ^= 2u, ty_bias = 3.0f * ty_coef - ty_bias;
+  if (dz > 0.0f) octant_mask ^= 4u, tz_bias = 3.0f * tz_coef - tz_bias;
+
+  float t_min = cl_fmax(cl_fmax(2.0f * tx_coef - tx_bias, 2.0f * ty_coef - ty_bias), 2.0f * tz_coef - tz_bias);
+  float t_max = cl_fmin(cl_fmin(tx_coef - tx_bias, ty_coef - ty_bias), tz_coef - tz_bias);
+  float h = t_max;
+  t_min = cl_fmax(t_min, 0.0f);
+
+  uint32_t parent = root_node;
+  uint32_t child_descriptor = 0;
+  uint32_t idx = 0;
+  float posx = 1.0f, posy = 1.0f, posz = 1.0f;
+  int scale = kSMax - 1;
+  float scale_exp2 = 0.5f;
+
+  if (1.5f * tx_coef - tx_bias > t_min) idx ^= 1u, posx = 1.5f;
+  if (1.5f * ty_coef - ty_bias > t_min) idx ^= 2u, posy = 1.5f;
+  if (1.5f * tz_coef - tz_bias > t_min) idx ^= 4u, posz = 1.5f;
+
+  int iter = 0;
+  while (scale < kSMax) {
+    if (++iter > kIterLimit) {
+      res.error_code = 1;
+      res.hit_t = t_min;
+      res.hit_node = int(parent);
+      res.node_depth = kSMax - scale - 1;
+      return;
+    }
+    if (child_descriptor == 0) {
+      child_descriptor = node_links[parent];
+      if (COUNT) ++fetches;
+    }
+    const float tx_corner = posx * tx_coef - tx_bias;
+    const float ty_corner = posy * ty_coef - ty_bias;
+    const float tz_corner = posz * tz_coef - tz_bias;
+    const float tc_max = cl_fmin(cl_fmin(tx_corner, ty_corner), tz_corner);
+
+    const uint32_t child_shift = idx ^ octant_mask;
+    const uint32_t child_masks = 1u << child_shift;
+    if ((child_descriptor & child_masks) != 0 && t_min <= t_max) {
+      if (child_descriptor & (1u << 8)) {
+        res.hit_node = int(parent);
+        res.hit_t = t_min;
+        res.fault_block = int(child_descriptor >> 11);
+        res.node_depth = kSMax - scale - 1;
+        return;
+      }
+      uint32_t ofs;
+      if (child_descriptor & (1u << 10)) {
+        ofs = far_pointers[child_descriptor >> 11];
+        if (COUNT) ++fetches;
+      } else {
+        ofs = (child_descriptor >> 11) + parent - (1u << 20);
+      }
+      ofs += __popc(child_descriptor & (child_masks - 1u) & 255u);
+      child_descriptor = node_links[ofs];
+      if (COUNT) ++fetches;
+
+      if (!(child_descriptor & 255u) || tc_max * ray_size_coef + ray_size_bias >= scale_exp2) {
+        res.hit_t = t_min;
+        res.hit_node = int(ofs);
+        res.hit_duplicate = int((child_descriptor >> 9) & 1u);
+        res.node_depth = kSMax - scale;
+        return;
+      }
+
+      const float half = scale_exp2 * 0.5f;
+      const float tx_center = half * tx_coef + tx_corner;
+      const float ty_center = half * ty_coef + ty_corner;
+      const float tz_center = half * tz_coef + tz_corner;
+
+      if (tc_max < h) stack.put(scale, make_uint2(parent, __float_as_uint(t_max)));
+      h = tc_max;
+      parent = ofs;
+
+      idx = 0;
+      scale--;
+      scale_exp2 = half;
+      if (tx_center > t_min) idx ^= 1u, posx += scale_exp2;
+      if (ty_center > t_min) idx ^= 2u, posy += scale_exp2;
+      if (tz_center > t_min) idx ^= 4u, posz += scale_exp2;
+      t_max = tc_max;
+      continue;
+    }
+
+    uint32_t step_mask = 0;
+    if (tx_corner <= tc_max) step_mask ^= 1u, posx -= scale_exp2;
+    if (ty_corner <= tc_max) step_mask ^= 2u, posy -= scale_exp2;
+    if (tz_corner <= tc_max) step_mask ^= 4u, posz -= scale_exp2;
+    t_min = tc_max;
+    idx ^= step_mask;
+
+    if ((idx & step_mask) != 0) {
+      uint32_t differing_bits = 0;
+      if (step_mask & 1u) differing_bits |= __float_as_uint(posx) ^ __float_as_uint(posx + scale_exp2);
+      if (step_mask & 2u) differing_bits |= __float_as_uint(posy) ^ __float_as_uint(posy + scale_exp2);
+      if (step_mask & 4u) differing_bits |= __float_as_uint(posz) ^ __float_as_uint(posz + scale_exp2);
+      // Highest set bit. The reference reads it from the exponent of
+      // (float)differing_bits; the two agree whenever the result is below
+      // S_MAX (the value is then exact in fp32), and both leave the loop
+      // otherwise.
+      scale = 31 - __clz(differing_bits);
+      if (scale >= kSMax) break;  // left the root cube
+      scale_exp2 = __uint_as_float(uint32_t(scale - kSMax + 127) << 23);
+
+      const uint2 e = stack.get(scale);
+      parent = e.x;
+      t_max = __uint_as_float(e.y);
+
+      const uint32_t shx = __float_as_uint(posx) >> scale;
+      const uint32_t shy = __float_as_uint(posy) >> scale;
+      const uint32_t shz = __float_as_uint(posz) >> scale;
+      posx = __uint_as_float(shx << scale);
+      posy = __uint_as_float(shy << scale);
+      posz = __uint_as_float(shz << scale);
+      idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
+      h = 0.0f;
+      child_descriptor = 0;
+    }
+  }
+  res.hit_t = 1e30f;
+  res.hit_node = -1;
+}
+
+// mul(float16, float3) of init_tracing_frame.cl:3-9: (x,y,z,1) against rows,
+// divided by the fourth row's dot product; the w output is 0/d.
+__device__ __forceinline__ float4 MulPoint(const float* m, float x, float y, float z) {
+  const float d = ((x * m[12] + y * m[13]) + z * m[14]) + 1.0f * m[15];
+  const float r0 = ((x * m[0] + y * m[1]) + z * m[2]) + 1.0f * m[3];
+  const float r1 = ((x * m[4] + y * m[5]) + z * m[6]) + 1.0f * m[7];
+  const float r2 = ((x * m[8] + y * m[9]) + z * m[10]) + 1.0f * m[11];
+  return make_float4(r0 / d, r1 / d, r2 / d, 0.0f / d);
+}
+
+__device__ __forceinline__ float Dot4(float4 a, float4 b) {
+  return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w;
+}
+
+struct PrimaryRay {
+  float4 origin;     // xyz origin, w = ray size bias
+  float4 direction;  // xyz direction, w = ray size coefficient
+};
+
+// InitTracingFrame for pixel (x, y) of a W x H view.
+__device__ __forceinline__ PrimaryRay InitRay(const float* m, int x, int y, int W, int H, float lod_voxel_size) {
+  const float fw = float(W), fh = float(H);
+  const float u = (float(x) + 0.5f) / fw * 2.0f - 1.0f;
+  const float v = (float(y) + 0.5f) / fh * 2.0f - 1.0f;
+  const float4 origin = make_float4(m[2] / m[14], m[6] / m[14], m[10] / m[14], 0.0f / m[14]);
+  const float4 p = MulPoint(m, u, v, 0.0f);
+  float4 dir = make_float4(p.x - origin.x, p.y - origin.y, p.z - origin.z, p.w - origin.w);
+  const float len = sqrtf(Dot4(dir, dir));
+  dir = make_float4(dir.x / len, dir.y / len, dir.z / len, dir.w / len);
+
+  const float4 p1 = MulPoint(m, float(x) / fw * 2.0f - 1.0f, float(y) / fh * 2.0f - 1.0f, 0.0f);
+  const float4 p2 = MulPoint(m, float(x + 1) / fw * 2.0f - 1.0f, float(y + 1) / fh * 2.0f - 1.0f, 0.0f);
+  const float4 d12 = make_float4(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, p1.w - p2.w);
+  const float4 d01 = make_float4(origin.x - p1.x, origin.y - p1.y, origin.z - p1.z, origin.w - p1.w);
+  const float coef = lod_voxel_size * sqrtf(Dot4(d12, d12)) / sqrtf(Dot4(d01, d01));
+
+  PrimaryRay r;
+  r.origin = make_float4(origin.x, origin.y, origin.z, 0.0f);
+  r.direction = make_float4(dir.x, dir.y, dir.z, coef);
+  return r;
+}
+
+// Light vectors of shading.cl:8-10, each normalize(v) * k with the pinned
+// normalize; computed once per kernel argument set on the host and passed in.
+struct Lights {
+  float l1[3], l2[3], l3[3];
+  float shadow_dir[3];  // unit direction towards light 1 (= -normalize(2,-2,3))
+};
+
+__device__ __forceinline__ uint32_t PackRGBA(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+  return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// Light-1 term (before the max) for the char4 normal; used to decide whether a
+// shadow ray is worth casting.
+__device__ __forceinline__ float LightTerm(float nx, float ny, float nz, const float* L) {
+  return (((-nx) * L[0] + (-ny) * L[1]) + (-nz) * L[2]) + (-0.0f) * 0.0f;
+}
+
+// ProcessHit colour. `occluded1` drops light 1 (shadow-ray extension; false
+// reproduces the reference exactly).
+__device__ __forceinline__ uint32_t ShadeColor(uint32_t cnormal, uint32_t ccolor, const Lights& lt, bool occluded1) {
+  const float nx = float(int8_t(cnormal & 255u)) / 127.f;
+  const float ny = float(int8_t((cnormal >> 8) & 255u)) / 127.f;
+  const float nz = float(int8_t((cnormal >> 16) & 255u)) / 127.f;
+  const float m1 = occluded1 ? 0.0f : cl_max(0.f, LightTerm(nx, ny, nz, lt.l1));
+  const float m2 = cl_max(0.f, LightTerm(nx, ny, nz, lt.l2));
+  const float m3 = cl_max(0.f, LightTerm(nx, ny, nz, lt.l3));
+  const float light = ((0.1f + m1 * 1.0f) + m2 * 1.0f) + m3 * 1.0f;
+  const float cr = cl_min(255.f, float(ccolor & 255u) * light);
+  const float cg = cl_min(255.f, float((ccolor >> 8) & 255u) * light);
+  const float cb = cl_min(255.f, float((ccolor >> 16) & 255u) * light);
+  return PackRGBA(uint32_t(cr), uint32_t(cg), uint32_t(cb), 255u);
+}
+
+// FinishTracingFrame blend of colour (RGBA8) over the background.
+__device__ __forceinline__ uint32_t Blend(uint32_t color, const float* bg) {
+  const float w = float(color >> 24);
+  const float k = 255.f - w;
+  const float tr = (float(color & 255u) + bg[0] * k) + 0.5f;
+  const float tg = (float((color >> 8) & 255u) + bg[1] * k) + 0.5f;
+  const float tb = (float((color >> 16) & 255u) + bg[2] * k) + 0.5f;
+  const float ta = (w + bg[3] * k) + 0.5f;
+  return PackRGBA(uint32_t(cl_max(0.f, cl_min(255.f, tr))), uint32_t(cl_max(0.f, cl_min(255.f, tg))),
+                  uint32_t(cl_max(0.f, cl_min(255.f, tb))), uint32_t(cl_max(0.f, cl_min(255.f, ta))));
+}
+
+}  // namespace rtdev

@@ -1,0 +1,575 @@
+// Renderer kernels for gfx950 and their C-ABI launchers.
+//
+// Workgroups are 256 lanes covering a 16x16 pixel tile made of four wave64
+// 8x8 sub-tiles, so the 64 rays of a wave are spatially coherent and walk the
+// same octree nodes. Tiles are remapped so that each of the 8 XCDs (one L2
+// each) receives a contiguous band of tiles rather than every 8th tile.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rt_internal.h"
+#include "svo_device.h"
+#include "trace_common.h"
+
+using namespace rtdev;
+
+namespace rayt {
+
+Lights MakeLights() {
+  // shading.cl:8-10 with normalize(v) = v / sqrt(dot(v,v)) in fp32.
+  auto norm = [](float x, float y, float z, float k, float* out) {
+    const float s = sqrtf(((x * x + y * y) + z * z) + 0.0f * 0.0f);
+    out[0] = (x / s) * k;
+    out[1] = (y / s) * k;
+    out[2] = (z / s) * k;
+  };
+  Lights lt;
+  norm(2.f, -2.f, 3.f, .6f, lt.l1);
+  norm(-3.f, -3.f, -2.f, .5f, lt.l2);
+  norm(1.f, 2.5f, -1.f, .3f, lt.l3);
+  float u[3];
+  norm(2.f, -2.f, 3.f, 1.0f, u);
+  lt.shadow_dir[0] = -u[0];
+  lt.shadow_dir[1] = -u[1];
+  lt.shadow_dir[2] = -u[2];
+  return lt;
+}
+
+}  // namespace rayt
+
+namespace {
+
+struct Mat16 {
+  float m[16];
+};
+struct Vec4 {
+  float v[4];
+};
+
+// Maps a linear workgroup id to a tile so that consecutive tiles land on the
+// same XCD (workgroups are dispatched round-robin over the 8 XCDs).
+__device__ __forceinline__ uint32_t XcdTile(uint32_t g, uint32_t tiles) {
+  const uint32_t full = tiles & ~7u;
+  if (g >= full) return g;
+  const uint32_t per = full >> 3;
+  return (g & 7u) * per + (g >> 3);
+}
+
+__device__ __forceinline__ void TilePixel(uint32_t tile, uint32_t tiles_x, int& x, int& y) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t wave = t >> 6, lane = t & 63u;
+  x = int((tile % tiles_x) * 16u + (wave & 1u) * 8u + (lane & 7u));
+  y = int((tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3));
+}
+
+__device__ __forceinline__ uint32_t LoadU32(const uint32_t* p, int64_t i, uint32_t fallback) {
+  return p ? p[i] : fallback;
+}
+
+// Colour of ProcessHit for `node`, optionally with light 1 occluded.
+__device__ __forceinline__ uint32_t ShadeNode(const uint32_t* normals, const uint32_t* colors, int node,
+                                              const Lights& lt, bool occluded) {
+  const uint32_t n = LoadU32(normals, node, 0u);
+  const uint32_t c = LoadU32(colors, node, 0xffffffffu);
+  return ShadeColor(n, c, lt, occluded);
+}
+
+// Shadow-ray test for a primary hit (extension; no reference counterpart).
+// Cast from just before the hit point (a quarter of the hit voxel back along
+// the primary ray, which is empty space) towards light 1, without LOD.
+template <class Stack, bool COUNT>
+__device__ __forceinline__ bool ShadowOccluded(const uint32_t* links, const uint32_t* far_ptrs, uint32_t root,
+                                               float ox, float oy, float oz, float dx, float dy, float dz,
+                                               const CastResult& hit, const Lights& lt, Stack& stack,
+                                               uint32_t& fetches, uint32_t& shadow_rays) {
+  const int nd = hit.node_depth;
+  const float eps = __uint_as_float(uint32_t(127 - nd - 2) << 23);
+  const float tb = cl_max(0.0f, hit.hit_t - eps);
+  const float sx = ox + dx * tb, sy = oy + dy * tb, sz = oz + dz * tb;
+  CastResult s;
+  CastRay<Stack, COUNT>(links, far_ptrs, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
+                        0.0f, 0.0f, stack, s, fetches);
+  if (COUNT) ++shadow_rays;
+  return s.error_code == 0 && s.fault_block == -1 && s.hit_node != -1;
+}
+
+__device__ __forceinline__ int HitCode(int hit_node, int dup, int nodes_in_block) {
+  const int unit = nodes_in_block > 0 ? hit_node / nodes_in_block : hit_node;
+  return ((unit + 1) << 1) | dup;
+}
+
+// ------------------------------------------------------------ InitTracingFrame
+__global__ __launch_bounds__(256) void InitTracingFrameKernel(float4* __restrict__ org, float4* __restrict__ dir,
+                                                              int4* __restrict__ misc, int W, int H, Mat16 m,
+                                                              float lod) {
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  if (x >= W || y >= H) return;
+  const int64_t i = int64_t(y) * W + x;
+  const PrimaryRay r = InitRay(m.m, x, y, W, H, lod);
+  org[i] = r.origin;
+  dir[i] = r.direction;
+  misc[i] = make_int4(-1, 0, int(0xffffffffu), 0);
+}
+
+// -------------------------------------------------------------- RaytracingPass
+template <int LDS_LEVELS>
+__global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ codes, float4* __restrict__ org,
+                                                            const float4* __restrict__ dir, int4* __restrict__ misc,
+                                                            int W, int H, const uint32_t* __restrict__ links,
+                                                            const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                            const uint32_t* __restrict__ normals,
+                                                            const uint32_t* __restrict__ colors, int nodes_in_block,
+                                                            int flags, Lights lt) {
+  __shared__ uint2 lds[LDS_LEVELS * 256];
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  if (x >= W || y >= H) return;
+  const int64_t i = int64_t(y) * W + x;
+  int4 s = misc[i];
+  if (uint32_t(s.z) == 0u) {
+    codes[i] = kNoHit;
+    return;
+  }
+  SplitStack<LDS_LEVELS, 256> stack;
+  stack.lds = lds + threadIdx.x;
+  float4 o = org[i];
+  const float4 d = dir[i];
+  CastResult res;
+  uint32_t fetches = 0, shadow_rays = 0;
+  CastRay<SplitStack<LDS_LEVELS, 256>, false>(links, far_ptrs, root, o.x, o.y, o.z, d.x, d.y, d.z, d.w, o.w, stack,
+                                              res, fetches);
+  int code;
+  if (res.error_code) {
+    s.y = int(PackRGBA(255, 0, 0, 255));
+    s.z = 0;
+    code = kNoHit;
+  } else if (res.fault_block != -1) {
+    o = make_float4(o.x + d.x * res.hit_t, o.y + d.y * res.hit_t, o.z + d.z * res.hit_t, o.w + d.w * res.hit_t);
+    org[i] = o;
+    s.x = res.hit_node;
+    code = -res.fault_block - 1;
+  } else if (res.hit_node == -1) {
+    s.z = 0;
+    code = kNoHit;
+  } else {
+    bool occluded = false;
+    if ((flags & RT_FLAG_SHADOWS) && normals) {
+      const uint32_t n = normals[res.hit_node];
+      const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
+                  nz = float(int8_t((n >> 16) & 255u)) / 127.f;
+      if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
+        occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, false>(links, far_ptrs, root, o.x, o.y, o.z, d.x, d.y,
+                                                                    d.z, res, lt, stack, fetches, shadow_rays);
+    }
+    s.y = int(ShadeNode(normals, colors, res.hit_node, lt, occluded));
+    s.z = 0;
+    code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);
+  }
+  misc[i] = s;
+  codes[i] = code;
+}
+
+// ---------------------------------------------------------- FinishTracingFrame
+__global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __restrict__ out,
+                                                                const int4* __restrict__ misc, int W, int H, Vec4 bg,
+                                                                const uint32_t* __restrict__ normals,
+                                                                const uint32_t* __restrict__ colors, Lights lt) {
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  if (x >= W || y >= H) return;
+  const int64_t i = int64_t(y) * W + x;
+  const int4 s = misc[i];
+  uint32_t color = uint32_t(s.y);
+  if (uint32_t(s.z) != 0u) color = s.x >= 0 ? ShadeNode(normals, colors, s.x, lt, false) : 0u;
+  out[i] = Blend(color, bg.v);
+}
+
+// ---------------------------------------------------------------- plain fused
+template <int LDS_LEVELS, bool COUNT>
+__global__ __launch_bounds__(256) void RenderPlainKernel(uint32_t* __restrict__ out, int* __restrict__ codes, int W,
+                                                         int H, const uint32_t* __restrict__ links,
+                                                         const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                         const uint32_t* __restrict__ normals,
+                                                         const uint32_t* __restrict__ colors, int nodes_in_block,
+                                                         Mat16 m, float lod, Vec4 bg, int flags, Lights lt,
+                                                         unsigned long long* __restrict__ stats) {
+  __shared__ uint2 lds[LDS_LEVELS * 256];
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  uint32_t fetches = 0, shadow_rays = 0, hits = 0, primaries = 0;
+  if (x < W && y < H) {
+    const int64_t i = int64_t(y) * W + x;
+    SplitStack<LDS_LEVELS, 256> stack;
+    stack.lds = lds + threadIdx.x;
+    const PrimaryRay r = InitRay(m.m, x, y, W, H, lod);
+    CastResult res;
+    CastRay<SplitStack<LDS_LEVELS, 256>, COUNT>(links, far_ptrs, root, r.origin.x, r.origin.y, r.origin.z,
+                                                r.direction.x, r.direction.y, r.direction.z, r.direction.w,
+                                                r.origin.w, stack, res, fetches);
+    if (COUNT) ++primaries;
+    uint32_t color;
+    int code;
+    if (res.error_code) {
+      color = PackRGBA(255, 0, 0, 255);
+      code = kNoHit;
+    } else if (res.fault_block != -1) {
+      // Unfinished after the single pass: FinishTracingFrame shades the fault parent.
+      color = ShadeNode(normals, colors, res.hit_node, lt, false);
+      code = -res.fault_block - 1;
+    } else if (res.hit_node == -1) {
+      color = 0u;
+      code = kNoHit;
+    } else {
+      bool occluded = false;
+      if ((flags & RT_FLAG_SHADOWS) && normals) {
+        const uint32_t n = normals[res.hit_node];
+        const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
+                    nz = float(int8_t((n >> 16) & 255u)) / 127.f;
+        if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
+          occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, COUNT>(
+              links, far_ptrs, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
+              r.direction.z, res, lt, stack, fetches, shadow_rays);
+      }
+      color = ShadeNode(normals, colors, res.hit_node, lt, occluded);
+      code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);
+      if (COUNT) ++hits;
+    }
+    out[i] = Blend(color, bg.v);
+    if (codes) codes[i] = code;
+  }
+  if (COUNT) {
+    __shared__ unsigned long long acc[4];
+    if (threadIdx.x < 4) acc[threadIdx.x] = 0;
+    __syncthreads();
+    atomicAdd(&acc[0], (unsigned long long)primaries);
+    atomicAdd(&acc[1], (unsigned long long)shadow_rays);
+    atomicAdd(&acc[2], (unsigned long long)fetches);
+    atomicAdd(&acc[3], (unsigned long long)hits);
+    __syncthreads();
+    if (threadIdx.x < 4) atomicAdd(&stats[threadIdx.x], acc[threadIdx.x]);
+  }
+}
+
+// ----------------------------------------------------- RaytraceKernel (depth)
+__global__ __launch_bounds__(256) void RaytraceDepthKernel(uint32_t* __restrict__ out, int W, int H,
+                                                           const uint32_t* __restrict__ links,
+                                                           const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                           Mat16 m, float lod) {
+  __shared__ uint2 lds[kStackLevels * 256];
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  if (x >= W || y >= H) return;
+  const float fw = float(W), fh = float(H);
+  const float u = (float(x) + 0.5f) / fw * 2.0f - 1.0f, v = (float(y) + 0.5f) / fh * 2.0f - 1.0f;
+  const float4 origin = MulPoint(m.m, u, v, -1.0f);
+  const float4 pf = MulPoint(m.m, u, v, 0.0f);
+  float4 d = make_float4(pf.x - origin.x, pf.y - origin.y, pf.z - origin.z, pf.w - origin.w);
+  const float len = sqrtf(Dot4(d, d));
+  d = make_float4(d.x / len, d.y / len, d.z / len, d.w / len);
+  const float x0 = float(x) / fw * 2.0f - 1.0f, y0 = float(y) / fh * 2.0f - 1.0f;
+  const float x1 = float(x + 1) / fw * 2.0f - 1.0f, y1 = float(y + 1) / fh * 2.0f - 1.0f;
+  const float4 p1n = MulPoint(m.m, x0, y0, -1.0f), p1f = MulPoint(m.m, x0, y0, 0.0f);
+  const float4 p2n = MulPoint(m.m, x1, y1, -1.0f), p2f = MulPoint(m.m, x1, y1, 0.0f);
+  auto dist = [](float4 a, float4 b) {
+    float4 t = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+    return sqrtf(Dot4(t, t));
+  };
+  const float bias = lod * dist(p1n, p2n);
+  const float coef = (bias < 1e-10f) ? 0.0f : (lod * (dist(p1f, p2f) - dist(p1n, p2n)) / dist(p1n, p1f));
+  SplitStack<kStackLevels, 256> stack;
+  stack.lds = lds + threadIdx.x;
+  CastResult res;
+  uint32_t fetches = 0;
+  CastRay<SplitStack<kStackLevels, 256>, false>(links, far_ptrs, root, origin.x, origin.y, origin.z, d.x, d.y, d.z,
+                                                coef, bias, stack, res, fetches);
+  const int64_t i = int64_t(y) * W + x;
+  if (res.error_code || res.fault_block != -1) {
+    out[i] = PackRGBA(255, 0, 0, 255);
+  } else if (res.hit_node == -1) {
+    out[i] = PackRGBA(0, 0, 0, 255);
+  } else {
+    out[i] = DepthGrey(res.hit_t);
+  }
+}
+
+// ------------------------------------------- legacy restart traversal (volatile)
+__global__ __launch_bounds__(256) void RaytraceRestartKernel(uint32_t* __restrict__ out, int W, int H,
+                                                             const uint32_t* __restrict__ links, uint32_t root,
+                                                             Mat16 m, float lod) {
+  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
+  int x, y;
+  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+  if (x >= W || y >= H) return;
+  const float fw = float(W), fh = float(H);
+  const float u = (float(x) + 0.5f) / fw * 2.0f - 1.0f, v = (float(y) + 0.5f) / fh * 2.0f - 1.0f;
+  const float4 origin = MulPoint(m.m, u, v, -1.0f);
+  const float4 pf = MulPoint(m.m, u, v, 0.0f);
+  float4 d = make_float4(pf.x - origin.x, pf.y - origin.y, pf.z - origin.z, pf.w - origin.w);
+  const float len = sqrtf(Dot4(d, d));
+  d = make_float4(d.x / len, d.y / len, d.z / len, d.w / len);
+  const float x0 = float(x) / fw * 2.0f - 1.0f, y0 = float(y) / fh * 2.0f - 1.0f;
+  const float x1 = float(x + 1) / fw * 2.0f - 1.0f, y1 = float(y + 1) / fh * 2.0f - 1.0f;
+  const float4 p1n = MulPoint(m.m, x0, y0, -1.0f), p1f = MulPoint(m.m, x0, y0, 0.0f);
+  const float4 p2n = MulPoint(m.m, x1, y1, -1.0f), p2f = MulPoint(m.m, x1, y1, 0.0f);
+  auto dist = [](float4 a, float4 b) {
+    float4 t = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+    return sqrtf(Dot4(t, t));
+  };
+  const float bias = lod * dist(p1n, p2n);
+  const float coef = (bias < 1e-10f) ? 0.0f : (lod * (dist(p1f, p2f) - dist(p1n, p2n)) / dist(p1n, p1f));
+  RestartResult res = CastRayRestart(links, root, origin.x, origin.y, origin.z, d.x, d.y, d.z, coef, bias);
+  const int64_t i = int64_t(y) * W + x;
+  if (res.error_code) {
+    out[i] = PackRGBA(255, 0, 0, 255);
+  } else {
+    const float ht = cl_min(1.0f, res.hit_t / 5.0f);
+    const float g = 255.0f - ht * 255.0f;
+    const uint32_t gi = uint32_t(g);
+    out[i] = PackRGBA(gi, gi, gi, 255);
+  }
+}
+
+__global__ void LegacyLinksKernel(uint32_t* __restrict__ out, const uint32_t* __restrict__ links,
+                                  const uint32_t* __restrict__ far_ptrs, uint64_t n, int* __restrict__ overflow) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t l = links[i];
+  const uint32_t mask = l & 255u;
+  uint32_t o = 0;
+  if (mask) {
+    uint64_t target;
+    if (l & (1u << 8)) {
+      target = l >> 11;  // fault: child block index
+      o = (1u << 8);
+    } else if (l & (1u << 10)) {
+      target = far_ptrs[l >> 11];
+    } else {
+      target = uint64_t(int64_t(i) + int64_t(l >> 11) - (1 << 20));
+    }
+    if (target >= (1ull << 23)) {
+      atomicOr(overflow, 1);
+      target = 0;
+    }
+    o |= uint32_t(target << 9) | mask;
+  }
+  out[i] = o;
+}
+
+int CheckTraceArgs(rt_ctx* ctx, int W, int H) {
+  RT_REQUIRE(ctx, RT_ERR_INVALID, "null context");
+  RT_REQUIRE(W > 0 && H > 0 && int64_t(W) * H <= (int64_t(1) << 30), RT_ERR_RANGE, "bad frame size");
+  return RT_OK;
+}
+
+uint32_t TileGrid(int W, int H) { return uint32_t((W + 15) / 16) * uint32_t((H + 15) / 16); }
+
+const uint32_t* U32(const rt_buf* b) { return b ? static_cast<const uint32_t*>(b->ptr) : nullptr; }
+
+template <int L>
+void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc, int W, int H,
+                 const uint32_t* links, const uint32_t* fp, uint32_t root, const uint32_t* nrm, const uint32_t* col,
+                 int nib, int flags, const Lights& lt) {
+  RaytracingPassKernel<L><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, links, fp, root, nrm, col, nib, flags,
+                                                  lt);
+}
+
+template <int L, bool C>
+void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const uint32_t* links,
+                  const uint32_t* fp, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
+                  const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, unsigned long long* st) {
+  RenderPlainKernel<L, C><<<g, dim3(256), 0, s>>>(out, codes, W, H, links, fp, root, nrm, col, nib, m, lod, bg,
+                                                  flags, lt, st);
+}
+
+template <bool C>
+bool DispatchPlain(int levels, dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H,
+                   const uint32_t* links, const uint32_t* fp, uint32_t root, const uint32_t* nrm,
+                   const uint32_t* col, int nib, const Mat16& m, float lod, const Vec4& bg, int flags,
+                   const Lights& lt, unsigned long long* st) {
+  switch (levels) {
+#define RT_CASE(L) \
+  case L: LaunchPlainL<L, C>(g, s, out, codes, W, H, links, fp, root, nrm, col, nib, m, lod, bg, flags, lt, st); return true;
+    RT_CASE(8) RT_CASE(10) RT_CASE(12) RT_CASE(14) RT_CASE(16) RT_CASE(22)
+#undef RT_CASE
+    default:
+      return false;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_init_tracing_frame(rt_ctx* ctx, rt_buf* states, int width, int height, const float vpi[16],
+                          float lod_voxel_size) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(states && vpi, RT_ERR_INVALID, "null argument");
+  const uint64_t P = uint64_t(width) * height;
+  RT_REQUIRE(states->size >= P * RT_STATE_BYTES_PER_PIXEL, RT_ERR_RANGE, "state buffer too small");
+  Mat16 m;
+  memcpy(m.m, vpi, sizeof(m.m));
+  char* base = static_cast<char*>(states->ptr);
+  InitTracingFrameKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
+      reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + 16 * P), reinterpret_cast<int4*>(base + 32 * P),
+      width, height, m, lod_voxel_size);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int width, int height,
+                       const rt_buf* node_links, const rt_buf* far_pointers, int root_node_index,
+                       const rt_buf* normals, const rt_buf* colors, int nodes_in_block, int flags) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(faults_and_hits && states && node_links && far_pointers, RT_ERR_INVALID, "null argument");
+  const uint64_t P = uint64_t(width) * height;
+  RT_REQUIRE(states->size >= P * RT_STATE_BYTES_PER_PIXEL && faults_and_hits->size >= P * 4, RT_ERR_RANGE,
+             "buffer too small");
+  RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
+             "root node outside pool");
+  RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
+  char* base = static_cast<char*>(states->ptr);
+  const Lights lt = rayt::MakeLights();
+  LaunchPassL<kStackLevels>(dim3(TileGrid(width, height)), ctx->stream, static_cast<int*>(faults_and_hits->ptr),
+                            reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + 16 * P),
+                            reinterpret_cast<int4*>(base + 32 * P), width, height, U32(node_links), U32(far_pointers),
+                            uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, flags, lt);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_finish_tracing_frame(rt_ctx* ctx, rt_buf* out_image, const rt_buf* states, int width, int height,
+                            const float background[4], const rt_buf* normals, const rt_buf* colors) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(out_image && states && background, RT_ERR_INVALID, "null argument");
+  const uint64_t P = uint64_t(width) * height;
+  RT_REQUIRE(states->size >= P * RT_STATE_BYTES_PER_PIXEL && out_image->size >= P * 4, RT_ERR_RANGE,
+             "buffer too small");
+  Vec4 bg;
+  memcpy(bg.v, background, sizeof(bg.v));
+  const char* base = static_cast<const char*>(states->ptr);
+  FinishTracingFrameKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
+      static_cast<uint32_t*>(out_image->ptr), reinterpret_cast<const int4*>(base + 32 * P), width, height, bg,
+      U32(normals), U32(colors), rayt::MakeLights());
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_raytrace_depth(rt_ctx* ctx, rt_buf* out_image, int width, int height, const rt_buf* node_links,
+                      const rt_buf* far_pointers, int root_node_index, const float vpi[16], float lod_voxel_size) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(out_image && node_links && far_pointers && vpi, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(out_image->size >= uint64_t(width) * height * 4, RT_ERR_RANGE, "image buffer too small");
+  RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
+             "root node outside pool");
+  Mat16 m;
+  memcpy(m.m, vpi, sizeof(m.m));
+  RaytraceDepthKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
+      static_cast<uint32_t*>(out_image->ptr), width, height, U32(node_links), U32(far_pointers),
+      uint32_t(root_node_index), m, lod_voxel_size);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_raytrace_restart(rt_ctx* ctx, rt_buf* out_image, int width, int height, const rt_buf* legacy_links,
+                        int root_node_index, const float vpi[16], float lod_voxel_size) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(out_image && legacy_links && vpi, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(out_image->size >= uint64_t(width) * height * 4, RT_ERR_RANGE, "image buffer too small");
+  RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < legacy_links->size, RT_ERR_RANGE,
+             "root node outside pool");
+  Mat16 m;
+  memcpy(m.m, vpi, sizeof(m.m));
+  RaytraceRestartKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
+      static_cast<uint32_t*>(out_image->ptr), width, height, U32(legacy_links), uint32_t(root_node_index), m,
+      lod_voxel_size);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_legacy_links_from_cache(rt_ctx* ctx, rt_buf* legacy_links, const rt_buf* node_links,
+                               const rt_buf* far_pointers, uint64_t node_count) {
+  RT_REQUIRE(ctx && legacy_links && node_links && far_pointers, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(node_count > 0 && node_count * 4 <= legacy_links->size && node_count * 4 <= node_links->size,
+             RT_ERR_RANGE, "node count exceeds buffers");
+  int* d_over = nullptr;
+  RT_HIP_CHECK(hipMalloc(&d_over, sizeof(int)));
+  hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream);
+  LegacyLinksKernel<<<dim3(unsigned((node_count + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+      static_cast<uint32_t*>(legacy_links->ptr), U32(node_links), U32(far_pointers), node_count, d_over);
+  int over = 0;
+  hipError_t e = hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d_over);
+  if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
+  RT_REQUIRE(!over, RT_ERR_RANGE, "pool needs more than 23-bit node indices for the legacy layout");
+  return RT_OK;
+}
+
+int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int tile_shape) {
+  RT_REQUIRE(ctx, RT_ERR_INVALID, "null context");
+  RT_REQUIRE(lds_stack_levels == 8 || lds_stack_levels == 10 || lds_stack_levels == 12 || lds_stack_levels == 14 ||
+                 lds_stack_levels == 16 || lds_stack_levels == 22,
+             RT_ERR_RANGE, "lds_stack_levels must be one of 8,10,12,14,16,22");
+  RT_REQUIRE(tile_shape == 0, RT_ERR_RANGE, "tile_shape must be 0 (8x8 wave tiles)");
+  ctx->plain_lds_levels = lds_stack_levels;
+  ctx->plain_tile = tile_shape;
+  return RT_OK;
+}
+
+int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, int height, const rt_buf* node_links,
+                    const rt_buf* far_pointers, int root_node_index, const rt_buf* normals, const rt_buf* colors,
+                    int nodes_in_block, const float vpi[16], float lod_voxel_size, const float background[4],
+                    int flags, uint64_t* stats) {
+  int rc = CheckTraceArgs(ctx, width, height);
+  if (rc) return rc;
+  RT_REQUIRE(out_image && node_links && far_pointers && vpi && background, RT_ERR_INVALID, "null argument");
+  const uint64_t P = uint64_t(width) * height;
+  RT_REQUIRE(out_image->size >= P * 4 && (!codes || codes->size >= P * 4), RT_ERR_RANGE, "buffer too small");
+  RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
+             "root node outside pool");
+  RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
+  Mat16 m;
+  memcpy(m.m, vpi, sizeof(m.m));
+  Vec4 bg;
+  memcpy(bg.v, background, sizeof(bg.v));
+  const Lights lt = rayt::MakeLights();
+  const dim3 grid(TileGrid(width, height));
+  if (flags & RT_FLAG_COUNT) {
+    RT_REQUIRE(stats, RT_ERR_INVALID, "counting render needs a stats array");
+    unsigned long long* d_stats = nullptr;
+    RT_HIP_CHECK(hipMalloc(&d_stats, 4 * sizeof(unsigned long long)));
+    hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream);
+    DispatchPlain<true>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, U32(node_links),
+                        U32(far_pointers), uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
+                        lod_voxel_size, bg, flags, lt, d_stats);
+    hipError_t e = hipGetLastError();
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_stats);
+    if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
+    for (int k = 0; k < 4; ++k) stats[k] = h[k];
+    return RT_OK;
+  }
+  DispatchPlain<false>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+                       codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, U32(node_links),
+                       U32(far_pointers), uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
+                       lod_voxel_size, bg, flags, lt, nullptr);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+}  // extern "C"

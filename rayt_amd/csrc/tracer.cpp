@@ -1,0 +1,229 @@
+// Frame renderer: GPURayTracer (GPU_renderer/gpu-ray-tracer.cpp:20-227).
+//
+// RenderFrame runs InitTracingFrame, then tracing passes each followed by
+// feedback extraction and one cache transaction (mark hit blocks, request
+// faulted blocks, upload), until a pass faults nothing, the session has
+// cycled the whole cache, or the pass budget is spent; FinishTracingFrame
+// then composes the image. When the cache holds every block of the tree no
+// ray can fault, and the single pass plus finish run as one fused kernel.
+#include <chrono>
+#include <cstring>
+#include <limits>
+
+#include "cache.h"
+#include "feedback.h"
+#include "tree_handle.h"
+
+struct rt_tracer {
+  rt_cache* cache = nullptr;
+  rt_ctx* ctx = nullptr;
+  int width = 0, height = 0;
+  double frame_time_limit = 1.0;  // gpu-ray-tracer.cpp:30
+  float lod_voxel_size = 1.5f;    // gpu-ray-tracer.cpp:31
+  int max_passes = 0;
+  int flags = 0;
+  rt_buf* out_image = nullptr;
+  rt_buf* codes = nullptr;
+  rt_buf* states = nullptr;
+  rt_feedback* feedback = nullptr;
+  uint64_t stats[5] = {0, 0, 0, 0, 0};
+  std::vector<int> pass_counts, pass_faults;
+  // HIP events around every tracing-pass launch of the last frame, and the
+  // summed device time between them.
+  std::vector<hipEvent_t> ev;
+  double trace_ms = 0.0, frame_ms = 0.0;
+  hipEvent_t Event(size_t i) {
+    while (ev.size() <= i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+      ev.push_back(e);
+    }
+    return ev[i];
+  }
+};
+
+namespace {
+
+const float kBackground[4] = {0.0f, float(216 / 255.), 1.0f, 1.0f};  // gpu-ray-tracer.cpp:190
+
+int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool& done) {
+  rayt::FeedbackExtractor& fx = *t->feedback->fx;
+  int rc = fx.Process(static_cast<int*>(t->codes->ptr), m.nodes_in_block(), true);
+  if (rc) return rc;
+  rc = m.StartRequestTransaction();
+  if (rc) return rc;
+  for (int h : fx.hit_blocks())
+    if ((rc = m.MarkBlockAsUsed(h))) return rc;
+  for (int h : fx.duplicate_hit_blocks())
+    if ((rc = m.MarkParentAsUsed(h))) return rc;
+  const std::vector<int>& faults = fx.fault_blocks();
+  for (size_t i = 0; i < faults.size(); ++i) {
+    if (m.TransactionFilledCache()) break;
+    if ((rc = m.RequestBlock(faults[i]))) return rc;
+  }
+  t->pass_counts.push_back(int(faults.size()));
+  t->pass_faults.insert(t->pass_faults.end(), faults.begin(), faults.end());
+  t->stats[1] += faults.size();
+  const uint64_t before = m.cache().uploaded;
+  rc = m.EndRequestTransaction();
+  if (rc) return rc;
+  t->stats[2] += m.cache().uploaded - before;
+  done = faults.empty() || m.SessionFilledCache();
+  return RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
+  RT_REQUIRE(cache && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(width > 0 && height > 0 && int64_t(width) * height <= (int64_t(1) << 30), RT_ERR_RANGE,
+             "bad frame size");
+  rt_tracer* t = new rt_tracer();
+  t->cache = cache;
+  t->ctx = cache->ctx;
+  t->width = width;
+  t->height = height;
+  const uint64_t P = uint64_t(width) * height;
+  int rc = rt_buf_create(t->ctx, P * 4, &t->out_image);
+  if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes);
+  if (!rc) rc = rt_buf_create(t->ctx, P * RT_STATE_BYTES_PER_PIXEL, &t->states);
+  if (!rc) rc = rt_feedback_create(t->ctx, int(P), &t->feedback);
+  if (rc) {
+    rt_tracer_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return RT_OK;
+}
+
+int rt_tracer_destroy(rt_tracer* t) {
+  if (!t) return RT_OK;
+  rt_buf_destroy(t->out_image);
+  rt_buf_destroy(t->codes);
+  rt_buf_destroy(t->states);
+  rt_feedback_destroy(t->feedback);
+  for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+  delete t;
+  return RT_OK;
+}
+
+int rt_tracer_set_frame_time_limit(rt_tracer* t, double s) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  RT_REQUIRE(s >= 0, RT_ERR_RANGE, "negative time limit");
+  t->frame_time_limit = s;
+  return RT_OK;
+}
+int rt_tracer_set_lod_voxel_size(rt_tracer* t, float size) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  RT_REQUIRE(size >= 0, RT_ERR_RANGE, "negative lod voxel size");
+  t->lod_voxel_size = size;
+  return RT_OK;
+}
+int rt_tracer_set_max_passes(rt_tracer* t, int passes) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  RT_REQUIRE(passes >= 0, RT_ERR_RANGE, "negative pass budget");
+  t->max_passes = passes;
+  return RT_OK;
+}
+int rt_tracer_set_flags(rt_tracer* t, int flags) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  t->flags = flags & RT_FLAG_SHADOWS;
+  return RT_OK;
+}
+int rt_tracer_set_seed(rt_tracer* t, uint64_t seed) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  t->feedback->fx->set_seed(seed);
+  return RT_OK;
+}
+
+int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
+  RT_REQUIRE(t && vpi, RT_ERR_INVALID, "null argument");
+  RT_TRY_BEGIN
+  const auto t0 = std::chrono::steady_clock::now();
+  rayt::BlockCacheManager& m = *t->cache->mgr;
+  rt_cache* c = t->cache;
+  for (uint64_t& s : t->stats) s = 0;
+  t->pass_counts.clear();
+  t->pass_faults.clear();
+  const int nib = m.nodes_in_block();
+  rt_buf* links = rt_cache_channel(c, 0);
+  rt_buf* far_ptrs = rt_cache_far_pointers(c);
+  rt_buf* normals = rt_cache_channel_by_name(c, "normals");
+  rt_buf* colors = rt_cache_channel_by_name(c, "colors");
+  const bool resident = uint32_t(m.cache().loaded_blocks_count()) == m.source()->header().blocks_count;
+  int rc;
+  if (!resident) {
+    rc = rt_init_tracing_frame(t->ctx, t->states, t->width, t->height, vpi, t->lod_voxel_size);
+    if (rc) return rc;
+  }
+  rc = m.StartRequestSession();
+  if (rc) return rc;
+  int iterations;
+  for (iterations = 0;; ++iterations) {
+    RT_HIP_CHECK(hipEventRecord(t->Event(2 * size_t(iterations)), t->ctx->stream));
+    if (resident)
+      rc = rt_render_plain(t->ctx, t->out_image, t->codes, t->width, t->height, links, far_ptrs,
+                           m.root_node_index(), normals, colors, nib, vpi, t->lod_voxel_size, kBackground, t->flags,
+                           nullptr);
+    else
+      rc = rt_raytracing_pass(t->ctx, t->codes, t->states, t->width, t->height, links, far_ptrs, m.root_node_index(),
+                              normals, colors, nib, t->flags);
+    if (rc) return rc;
+    RT_HIP_CHECK(hipEventRecord(t->Event(2 * size_t(iterations) + 1), t->ctx->stream));
+    if (iterations > 0) {
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if ((t->max_passes > 0 && iterations + 1 >= t->max_passes) || el > t->frame_time_limit) break;
+    }
+    bool done = false;
+    rc = FeedbackAndTransaction(t, m, done);
+    if (rc) return rc;
+    if (done) break;
+  }
+  t->stats[0] = uint64_t(iterations + 1);
+  t->stats[3] = m.SessionFilledCache() ? 1 : 0;
+  t->stats[4] = uint64_t(t->width) * t->height;
+  rc = m.EndRequestSession();
+  if (rc) return rc;
+  if (!resident) {
+    rc = rt_finish_tracing_frame(t->ctx, t->out_image, t->states, t->width, t->height, kBackground, normals, colors);
+    if (rc) return rc;
+  }
+  RT_HIP_CHECK(hipStreamSynchronize(t->ctx->stream));
+  t->trace_ms = 0.0;
+  for (int i = 0; i <= iterations; ++i) {
+    float ms = 0.f;
+    RT_HIP_CHECK(hipEventElapsedTime(&ms, t->ev[2 * i], t->ev[2 * i + 1]));
+    t->trace_ms += ms;
+  }
+  t->frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RT_OK;
+  RT_TRY_END
+}
+
+int rt_tracer_timings(const rt_tracer* t, double out_ms[2]) {
+  RT_REQUIRE(t && out_ms, RT_ERR_INVALID, "null argument");
+  out_ms[0] = t->trace_ms;
+  out_ms[1] = t->frame_ms;
+  return RT_OK;
+}
+
+rt_buf* rt_tracer_out_image(rt_tracer* t) { return t ? t->out_image : nullptr; }
+rt_buf* rt_tracer_codes(rt_tracer* t) { return t ? t->codes : nullptr; }
+rt_feedback* rt_tracer_feedback(rt_tracer* t) { return t ? t->feedback : nullptr; }
+
+int rt_tracer_frame_stats(const rt_tracer* t, uint64_t out[5]) {
+  RT_REQUIRE(t && out, RT_ERR_INVALID, "null argument");
+  memcpy(out, t->stats, sizeof(t->stats));
+  return RT_OK;
+}
+
+int rt_tracer_pass_faults(const rt_tracer* t, int* counts, int counts_cap, int* faults, int faults_cap) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  for (int i = 0; i < counts_cap && i < int(t->pass_counts.size()); ++i) counts[i] = t->pass_counts[i];
+  for (int i = 0; i < faults_cap && i < int(t->pass_faults.size()); ++i) faults[i] = t->pass_faults[i];
+  return int(t->pass_counts.size());
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+"""Seeded scenes and cameras shared by the tests (built through the product's
+builder; the oracle reads the same .tree bytes from disk)."""
+from __future__ import annotations
+
+import functools
+import os
+import tempfile
+
+import numpy as np
+
+from rayt_amd import api
+
+SCENE_DIR = os.path.join(tempfile.gettempdir(), "rayt_test_scenes")
+
+# The CPU-plain workload: depth-8 terrain (4 octaves) + 8 spheres, 640x480, 60 deg.
+TERRAIN8 = api.TerrainSpec(depth=8, seed=1, octaves=4, spheres=8)
+# The GPU-plain workload: depth-11 terrain + 64 boxes.
+TERRAIN11 = api.TerrainSpec(depth=11, seed=1, octaves=4, boxes=64)
+
+
+def camera_vpi(yaw, pitch, pos, fov=60.0, aspect=4.0 / 3.0, roll=0.0):
+    return api.Camera(yaw=yaw, pitch=pitch, roll=roll, position=pos, fov=fov, aspect=aspect).view_proj_inv()
+
+
+# Fixed poses: outside looking in, inside the cube, grazing, straight down, axis-aligned.
+POSES = [
+    (30.0, -20.0, (0.1, 0.9, 1.3)),
+    (-135.0, -10.0, (0.8, 0.7, 0.8)),
+    (90.0, -2.0, (0.5, 0.45, 0.5)),
+    (0.0, -89.0, (0.5, 1.5, 0.5)),
+    (0.0, 0.0, (0.5, 0.3, 1.9)),
+    (180.0, 30.0, (0.5, 0.1, -0.8)),
+]
+
+
+@functools.lru_cache(maxsize=None)
+def terrain_tree(depth: int, nodes_in_block: int, spheres: int = 0, boxes: int = 0, seed: int = 1, octaves: int = 4):
+    """(product tree handle, path of the same tree written as a .tree file)."""
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    spec = api.TerrainSpec(depth=depth, seed=seed, octaves=octaves, spheres=spheres, boxes=boxes)
+    tree = api.StoredOctree.terrain(spec, nodes_in_block)
+    path = os.path.join(SCENE_DIR, f"terrain_d{depth}_n{nodes_in_block}_s{spheres}_b{boxes}_r{seed}_o{octaves}.tree")
+    tree.write(path)
+    return tree, path
+
+
+@functools.lru_cache(maxsize=None)
+def sphere_tree(center, radius, level, nodes_in_block):
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    tree = api.StoredOctree.sphere(center, radius, level, nodes_in_block)
+    path = os.path.join(SCENE_DIR, f"sphere_{center}_{radius}_{level}_{nodes_in_block}.tree".replace(" ", ""))
+    tree.write(path)
+    return tree, path

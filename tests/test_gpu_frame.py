@@ -1,0 +1,73 @@
+"""GPU parity of the full RenderFrame loop (passes + feedback + cache
+transactions + finish) against the oracle's restatement of
+gpu-ray-tracer.cpp:66-227 on an undersized block pool, frame after frame:
+per-pass fault lists, the image, every pool word and the LRU order must be
+identical."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from rayt_amd import api
+from tests.scenes import POSES, camera_vpi, terrain_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(gpu_ctx, depth, nib, boxes, pool_frac, W, H, flags, max_passes, lod, poses, hits_to_extract=4000):
+    tree, path = terrain_tree(depth, nib, 0, boxes)
+    info = tree.info()
+    max_blocks = max(4, int(info.blocks_count * pool_frac))
+    m = api.GPUOctreeCacheManager(max_blocks, tree, gpu_ctx)
+    m.InitialFillCache()
+    tr = api.GPURayTracer(m, W, H)
+    tr.set_frame_time_limit(1e9)
+    tr.set_max_passes(max_passes)
+    tr.set_lod_voxel_size(lod)
+    tr.set_flags(flags)
+    tr.set_seed(99)
+    t = orc.Tree(path)
+    c = orc.Cache(t, max_blocks)
+    c.initial_fill()
+    assert m.root_node_index() == c.root()
+    rng, ef = 99, 1 << 16
+    total_uploads = 0
+    for k, (yaw, pitch, pos) in enumerate(poses):
+        vpi = camera_vpi(yaw, pitch, pos, aspect=W / H)
+        tr.RenderFrame(vpi)
+        r = c.render_frame(W, H, vpi, lod=lod, flags=flags, max_passes=max_passes, hits_to_extract=hits_to_extract,
+                           seed=rng, expected_faults=ef)
+        rng, ef = r["rng"], r["expected_faults"]
+        assert tr.pass_faults() == r["pass_faults"], f"fault lists differ in frame {k}"
+        st = tr.frame_stats()
+        assert st[0] == r["passes"]
+        np.testing.assert_array_equal(tr.out_image(), r["image"], err_msg=f"image differs in frame {k}")
+        pool = m.pool_arrays()
+        np.testing.assert_array_equal(pool["links"], c.links(), err_msg=f"pool links differ after frame {k}")
+        np.testing.assert_array_equal(pool["far"], c.far())
+        np.testing.assert_array_equal(pool["normals"], c.channel_bytes(1, 4))
+        assert m.lru_order() == c.lru_order(), f"LRU order differs after frame {k}"
+        total_uploads = m.counters()[0]
+    return total_uploads, m.counters()
+
+
+def test_frame_out_of_core_small_pool(gpu_ctx):
+    uploads, counters = _run(gpu_ctx, depth=9, nib=512, boxes=16, pool_frac=0.35, W=320, H=240, flags=0,
+                             max_passes=6, lod=0.0, poses=POSES[:4])
+    assert counters[1] > 0, "the pool was expected to evict blocks"
+
+
+def test_frame_out_of_core_shadows_lod(gpu_ctx):
+    _run(gpu_ctx, depth=9, nib=1024, boxes=16, pool_frac=0.5, W=256, H=192, flags=api.L.RT_FLAG_SHADOWS,
+         max_passes=4, lod=1.0, poses=POSES[2:6])
+
+
+def test_frame_tiny_pool_session_fills(gpu_ctx):
+    """A pool far smaller than the working set: transactions stop when the LRU marker is hit."""
+    _run(gpu_ctx, depth=8, nib=256, boxes=8, pool_frac=0.08, W=160, H=120, flags=0, max_passes=0, lod=0.0,
+         poses=POSES[:3], hits_to_extract=50)
+
+
+def test_frame_resident_single_pass(gpu_ctx):
+    """Whole tree resident: one pass (fused kernel), no faults, LRU touched by hits."""
+    _run(gpu_ctx, depth=8, nib=8192, boxes=0, pool_frac=1.0, W=320, H=240, flags=api.L.RT_FLAG_SHADOWS,
+         max_passes=0, lod=0.0, poses=POSES[:3])

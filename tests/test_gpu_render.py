@@ -1,0 +1,171 @@
+"""GPU parity: the HIP renderer, cache and feedback pipeline against the oracle.
+
+Hit node ids, feedback codes, pool words and RGBA8 pixels are compared
+bit-exactly; hit distances are compared through the pixels and codes they
+determine (the renderer does not export t).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from rayt_amd import api
+from tests.scenes import POSES, camera_vpi, sphere_tree, terrain_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_pool(path, max_blocks=None):
+    t = orc.Tree(path)
+    c = orc.Cache(t, max_blocks or t.blocks)
+    c.initial_fill()
+    return t, c
+
+
+def _gpu_cache(ctx, tree, max_blocks=None):
+    info = tree.info()
+    m = api.GPUOctreeCacheManager(max_blocks or info.blocks_count, tree, ctx)
+    m.InitialFillCache()
+    return m
+
+
+def _oracle_arrays(t, c):
+    links, far = c.links(), c.far()
+    normals = c.channel_bytes(1, 4).view(np.uint32)
+    colors = c.channel_bytes(2, 4).view(np.uint32)
+    return links, far, normals, colors
+
+
+@pytest.mark.parametrize("level,nib", [(3, 131), (5, 111), (7, 111), (6, 8)])
+def test_sphere_pool_layout(gpu_ctx, level, nib):
+    """CheckTestOctreeSphereWithGPUData on the device pool after InitialFillCache (whole tree)."""
+    center, radius = (0.3, 0.4, 0.65), 0.23
+    tree, path = sphere_tree(center, radius, level, nib)
+    m = _gpu_cache(gpu_ctx, tree)
+    pool = m.pool_arrays()
+    err = orc.check_sphere_pool(pool["links"], pool["far"], pool["node index"], pool["node index % 211"],
+                                m.root_node_index(), center, radius, level, faultless=True)
+    assert err == 0
+    t, c = _oracle_pool(path)
+    assert m.root_node_index() == c.root()
+    np.testing.assert_array_equal(pool["links"], c.links())
+    np.testing.assert_array_equal(pool["far"], c.far())
+    np.testing.assert_array_equal(pool["node index"], c.channel_bytes(1, 4))
+    np.testing.assert_array_equal(pool["node index % 211"], c.channel_bytes(2, 1))
+
+
+@pytest.mark.parametrize("depth,nib,spheres,boxes", [(8, 8192, 8, 0), (9, 1024, 0, 16), (10, 8192, 0, 64)])
+def test_terrain_pool_matches_oracle(gpu_ctx, depth, nib, spheres, boxes):
+    tree, path = terrain_tree(depth, nib, spheres, boxes)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    pool = m.pool_arrays()
+    assert m.root_node_index() == c.root()
+    np.testing.assert_array_equal(pool["links"], c.links())
+    np.testing.assert_array_equal(pool["far"], c.far())
+    np.testing.assert_array_equal(pool["normals"], c.channel_bytes(1, 4))
+    np.testing.assert_array_equal(pool["colors"], c.channel_bytes(2, 4))
+    assert m.lru_order() == c.lru_order()
+
+
+@pytest.mark.parametrize("lod", [0.0, 1.0])
+@pytest.mark.parametrize("flags", [0, api.L.RT_FLAG_SHADOWS])
+def test_render_plain_matches_oracle(gpu_ctx, lod, flags):
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    W, H = 320, 240
+    for yaw, pitch, pos in POSES:
+        vpi = camera_vpi(yaw, pitch, pos)
+        out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi, lod=lod, flags=flags)
+        img = out.read_all(np.uint32).reshape(H, W)
+        cod = codes.read_all(np.int32).reshape(H, W)
+        r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi, lod=lod, flags=flags)
+        np.testing.assert_array_equal(cod, r["codes"], err_msg=f"codes differ at pose {yaw, pitch, pos}")
+        np.testing.assert_array_equal(img, r["image"], err_msg=f"image differs at pose {yaw, pitch, pos}")
+
+
+def test_render_counts_match_oracle(gpu_ctx):
+    """The counting kernel's totals (rays, shadow rays, fetches F, hits) equal the oracle's."""
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    vpi = camera_vpi(*POSES[0])
+    _, _, st = api.render_plain(gpu_ctx, m, 640, 480, vpi, flags=api.L.RT_FLAG_SHADOWS, count=True)
+    r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, 640, 480, vpi,
+                         flags=api.L.RT_FLAG_SHADOWS)
+    assert st == r["stats"]
+
+
+def test_init_pass_finish_match_oracle(gpu_ctx):
+    """InitTracingFrame, one RaytracingPass (node-level codes) and FinishTracingFrame as separate kernels."""
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    W, H = 200, 150
+    P = W * H
+    lib = api.lib()
+    for yaw, pitch, pos in POSES[:3]:
+        vpi = camera_vpi(yaw, pitch, pos)
+        states = api.Buffer(gpu_ctx, P * 48)
+        api.call("rt_init_tracing_frame", gpu_ctx.h, states.h, W, H, api._fp(vpi), np.float32(1.5))
+        raw = states.read_all(np.uint8)
+        org = raw[:16 * P].view(np.float32).reshape(P, 4)
+        dr = raw[16 * P:32 * P].view(np.float32).reshape(P, 4)
+        misc = raw[32 * P:].view(np.int32).reshape(P, 4)
+        ost = orc.init_tracing_frame(W, H, vpi, 1.5).reshape(P, 12)
+        np.testing.assert_array_equal(org.view(np.uint32), ost[:, 0:4].view(np.uint32))
+        np.testing.assert_array_equal(dr.view(np.uint32), ost[:, 4:8].view(np.uint32))
+        np.testing.assert_array_equal(misc, ost[:, 8:12].view(np.int32))
+        codes = api.Buffer(gpu_ctx, P * 4)
+        nrm, col = m.channel_by_name("normals"), m.channel_by_name("colors")
+        api.call("rt_raytracing_pass", gpu_ctx.h, codes.h, states.h, W, H, m.channel(0).h, m.far_pointers().h,
+                 m.root_node_index(), nrm.h, col.h, 0, 0)
+        ocodes = orc.raytracing_pass(ost.reshape(-1), W, H, links, far, c.root(), normals, colors, 0, 0)
+        np.testing.assert_array_equal(codes.read_all(np.int32), ocodes)
+        out = api.Buffer(gpu_ctx, P * 4)
+        api.call("rt_finish_tracing_frame", gpu_ctx.h, out.h, states.h, W, H, api._fp(api.BACKGROUND), nrm.h, col.h)
+        np.testing.assert_array_equal(out.read_all(np.uint32),
+                                      orc.finish_tracing_frame(ost.reshape(-1), W, H, normals, colors))
+
+
+def test_depth_kernel_matches_oracle(gpu_ctx):
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, _, _ = _oracle_arrays(t, c)
+    W, H = 256, 192
+    for yaw, pitch, pos in POSES[:4]:
+        vpi = camera_vpi(yaw, pitch, pos)
+        out = api.Buffer(gpu_ctx, W * H * 4)
+        api.call("rt_raytrace_depth", gpu_ctx.h, out.h, W, H, m.channel(0).h, m.far_pointers().h,
+                 m.root_node_index(), api._fp(vpi), np.float32(1.0))
+        got = out.read_all(np.uint32)
+        exp, _ = orc.raytrace_depth(links, far, c.root(), W, H, vpi, 1.0)
+        # grey levels come from a double-precision log rounded to fp32; allow one level
+        g = got.view(np.uint8).reshape(-1, 4).astype(int)
+        e = exp.view(np.uint8).reshape(-1, 4).astype(int)
+        assert np.abs(g - e).max() <= 1
+        assert (g != e).any(axis=1).mean() < 1e-3
+
+
+def test_restart_kernel_matches_oracle(gpu_ctx):
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, _, _ = _oracle_arrays(t, c)
+    n = len(links)
+    legacy = api.Buffer(gpu_ctx, n * 4)
+    api.call("rt_legacy_links_from_cache", gpu_ctx.h, legacy.h, m.channel(0).h, m.far_pointers().h, n)
+    oleg = orc.legacy_links(links, far)
+    np.testing.assert_array_equal(legacy.read_all(np.uint32), oleg)
+    W, H = 256, 192
+    for yaw, pitch, pos in POSES[:4]:
+        vpi = camera_vpi(yaw, pitch, pos)
+        out = api.Buffer(gpu_ctx, W * H * 4)
+        api.call("rt_raytrace_restart", gpu_ctx.h, out.h, W, H, legacy.h, m.root_node_index(), api._fp(vpi),
+                 np.float32(1.0))
+        exp, _ = orc.raytrace_restart(oleg, c.root(), W, H, vpi, 1.0)
+        np.testing.assert_array_equal(out.read_all(np.uint32), exp)
