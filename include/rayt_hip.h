@@ -263,6 +263,37 @@ int rt_tracer_pass_faults(const rt_tracer* tr, int* counts, int counts_cap, int*
  * (the PROFILE_TIMER "GPU tracing passes" / "frame" of gpu-ray-tracer.cpp:124-263). */
 int rt_tracer_timings(const rt_tracer* tr, double out_ms[2]);
 
+/* ---------------------------------------------------------------- batched
+ * Many views traced by one persistent kernel (no reference counterpart: the
+ * reference renders one view per RenderFrame). Each view's pixels and
+ * feedback codes are those of rt_render_plain for that view, written at
+ * pixel_offset into shared output buffers. */
+typedef struct rt_view {
+  float view_proj_inv[16];
+  int width, height;
+  float lod_voxel_size;
+  int flags; /* RT_FLAG_SHADOWS */
+  uint64_t pixel_offset;
+} rt_view;
+
+int rt_batch_create(rt_cache* cache, int max_views, rt_batch** out);
+int rt_batch_destroy(rt_batch* batch);
+/* Launch shape: tile_shape 0 = 8x8, 1 = 16x4, 2 = 32x2 pixels per wave; lds_levels (8, 10, 12, 16) deepest
+ * traversal-stack levels kept in LDS; wgs_per_cu workgroups of 256 lanes per compute unit; tree_depth of the
+ * scene (places the LDS window on the deepest levels). */
+int rt_batch_config(rt_batch* batch, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth);
+/* Enqueues one batched launch on the cache's context stream. stats (may be NULL) makes it the counting variant
+ * and receives {primary rays, shadow rays, node fetches, hits}. */
+int rt_batch_render(rt_batch* batch, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes,
+                    uint64_t* stats);
+/* Device time of the last rt_batch_render launch (HIP events; waits for it). */
+int rt_batch_last_ms(rt_batch* batch, double* ms);
+/* Autotuner: times every feasible (tile shape, LDS stack levels, workgroups per CU) on this batch (median of
+ * `repeats` launches after one warm-up), keeps the fastest as the batch configuration and returns it in
+ * best = {tile_shape, lds_levels, wgs_per_cu} with its time. */
+int rt_batch_autotune(rt_batch* batch, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes,
+                      int repeats, int best[3], double* best_ms);
+
 /* ----------------------------------------------------------------- camera */
 /* Camera::ViewProjectionMatrix().Inverse(): scene/camera.cpp:23-28, util/mat.cpp:6-194. */
 int rt_camera_view_proj_inv(float yaw, float pitch, float roll, const float position[3], float fov_deg,

@@ -128,7 +128,18 @@ SIGNATURES = {
     "rt_tracer_pass_faults": (I, [P, IP, I, IP, I]),
     "rt_tracer_timings": (I, [P, C.POINTER(D)]),
     "rt_camera_view_proj_inv": (I, [F, F, F, FP, F, F, F, FP]),
+    "rt_batch_create": (I, [P, I, PP]),
+    "rt_batch_destroy": (I, [P]),
+    "rt_batch_config": (I, [P, I, I, I, I]),
+    "rt_batch_render": (I, [P, C.c_void_p, I, P, P, U64P]),
+    "rt_batch_last_ms": (I, [P, C.POINTER(D)]),
+    "rt_batch_autotune": (I, [P, C.c_void_p, I, P, P, I, IP, C.POINTER(D)]),
 }
+
+
+class View(C.Structure):
+    _fields_ = [("view_proj_inv", F * 16), ("width", I), ("height", I), ("lod_voxel_size", F), ("flags", I),
+                ("pixel_offset", U64)]
 
 _lib = None
 

@@ -466,3 +466,62 @@ def render_plain(ctx: Context, cache: GPUOctreeCacheManager, width: int, height:
          cache.nodes_in_block(), _fp(m), np.float32(lod), _fp(BACKGROUND), flags | (L.RT_FLAG_COUNT if count else 0),
          stats if count else None)
     return out, codes, (list(stats) if count else None)
+
+
+class BatchRenderer:
+    """Many views in one persistent launch (rt_batch_*)."""
+
+    TILE_SHAPES = {0: (8, 8), 1: (16, 4), 2: (32, 2)}
+
+    def __init__(self, cache: GPUOctreeCacheManager, max_views: int = 256):
+        h = C.c_void_p()
+        call("rt_batch_create", cache.h, max_views, C.byref(h))
+        self.h, self.cache, self.ctx = h, cache, cache.ctx
+
+    def config(self, tile_shape: int = 0, lds_levels: int = 12, wgs_per_cu: int = 4, tree_depth: int = 11):
+        call("rt_batch_config", self.h, tile_shape, lds_levels, wgs_per_cu, tree_depth)
+
+    @staticmethod
+    def make_views(specs, flags: int = 0, lod: float = 0.0):
+        """specs: list of (view_proj_inv, width, height). Returns (ctypes array, total pixels, offsets)."""
+        arr = (L.View * len(specs))()
+        off, offsets = 0, []
+        for i, (vpi, w, h) in enumerate(specs):
+            arr[i].view_proj_inv[:] = [float(x) for x in np.asarray(vpi, np.float32).reshape(16)]
+            arr[i].width, arr[i].height = w, h
+            arr[i].lod_voxel_size = lod
+            arr[i].flags = flags
+            arr[i].pixel_offset = off
+            offsets.append(off)
+            off += w * h
+        return arr, off, offsets
+
+    def render(self, views, nviews: int, out: Buffer, codes: Buffer | None = None, count: bool = False):
+        stats = (C.c_uint64 * 4)()
+        call("rt_batch_render", self.h, C.cast(views, C.c_void_p), nviews, out.h, codes.h if codes else None,
+             stats if count else None)
+        return list(stats) if count else None
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        call("rt_batch_last_ms", self.h, C.byref(ms))
+        return ms.value
+
+    def autotune(self, views, nviews: int, out: Buffer, codes: Buffer | None = None, repeats: int = 3):
+        """Picks the fastest (tile shape, LDS levels, workgroups/CU); returns (config tuple, ms)."""
+        best = (C.c_int * 3)()
+        ms = C.c_double()
+        call("rt_batch_autotune", self.h, C.cast(views, C.c_void_p), nviews, out.h, codes.h if codes else None,
+             repeats, best, C.byref(ms))
+        return (best[0], best[1], best[2]), ms.value
+
+    def close(self):
+        if self.h:
+            lib().rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

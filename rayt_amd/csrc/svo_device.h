@@ -50,30 +50,52 @@ struct CastResult {
 // is only touched for trees deeper than LDS_LEVELS+1.
 template <int LDS_LEVELS, int BLOCK>
 struct SplitStack {
-  uint2* lds;  // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
-  uint2 spill[(kStackLevels - LDS_LEVELS) > 0 ? (kStackLevels - LDS_LEVELS) : 1];
+  uint2* lds;    // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
+  int base = 0;  // first level held in LDS; levels outside [base, base + LDS_LEVELS) use `spill`
+  uint2 spill[LDS_LEVELS >= kStackLevels ? 1 : kStackLevels];
   __device__ __forceinline__ void put(int scale, uint2 v) {
-    int level = (kSMax - 1) - scale;
-    if (LDS_LEVELS >= kStackLevels || level < LDS_LEVELS)
+    const int level = (kSMax - 1) - scale;
+    if constexpr (LDS_LEVELS >= kStackLevels) {
       lds[level * BLOCK] = v;
-    else
-      spill[level - LDS_LEVELS] = v;
+    } else {
+      const int l = level - base;
+      if (unsigned(l) < unsigned(LDS_LEVELS))
+        lds[l * BLOCK] = v;
+      else
+        spill[level] = v;
+    }
   }
   __device__ __forceinline__ uint2 get(int scale) const {
-    int level = (kSMax - 1) - scale;
-    if (LDS_LEVELS >= kStackLevels || level < LDS_LEVELS) return lds[level * BLOCK];
-    return spill[level - LDS_LEVELS];
+    const int level = (kSMax - 1) - scale;
+    if constexpr (LDS_LEVELS >= kStackLevels) {
+      return lds[level * BLOCK];
+    } else {
+      const int l = level - base;
+      if (unsigned(l) < unsigned(LDS_LEVELS)) return lds[l * BLOCK];
+      return spill[level];
+    }
   }
+};
+
+// The block pool as seen by the traversal: node links and far pointers with
+// their element counts. A link that points outside either array ends the ray
+// with error code 2 (a corrupt pool never turns into a memory fault).
+struct PoolView {
+  const uint32_t* __restrict__ links;
+  const uint32_t* __restrict__ far_ptrs;
+  uint32_t nodes;
+  uint32_t far_count;
 };
 
 // ESVO traversal over the cache link format. `fetches` counts executed
 // node-link and far-pointer loads (the F of the roofline accounting).
 template <class Stack, bool COUNT>
-__device__ __forceinline__ void CastRay(const uint32_t* __restrict__ node_links,
-                                        const uint32_t* __restrict__ far_pointers, uint32_t root_node,
+__device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node,
                                         float px, float py, float pz, float dx, float dy, float dz,
                                         float ray_size_coef, float ray_size_bias, Stack& stack,
                                         CastResult& res, uint32_t& fetches) {
+  const uint32_t* __restrict__ node_links = pool.links;
+  const uint32_t* __restrict__ far_pointers = pool.far_ptrs;
   px += 1.0f;
   py += 1.0f;
   pz += 1.0f;
@@ -143,12 +165,26 @@ __device__ __forceinline__ void CastRay(const uint32_t* __restrict__ node_links,
       }
       uint32_t ofs;
       if (child_descriptor & (1u << 10)) {
+        if ((child_descriptor >> 11) >= pool.far_count) {
+          res.error_code = 2;
+          res.hit_t = t_min;
+          res.hit_node = int(parent);
+          res.node_depth = kSMax - scale - 1;
+          return;
+        }
         ofs = far_pointers[child_descriptor >> 11];
         if (COUNT) ++fetches;
       } else {
         ofs = (child_descriptor >> 11) + parent - (1u << 20);
       }
       ofs += __popc(child_descriptor & (child_masks - 1u) & 255u);
+      if (ofs >= pool.nodes || scale < 1) {
+        res.error_code = 2;
+        res.hit_t = t_min;
+        res.hit_node = int(parent);
+        res.node_depth = kSMax - scale - 1;
+        return;
+      }
       child_descriptor = node_links[ofs];
       if (COUNT) ++fetches;
 

@@ -4,6 +4,7 @@
 // 8x8 sub-tiles, so the 64 rays of a wave are spatially coherent and walk the
 // same octree nodes. Tiles are remapped so that each of the 8 XCDs (one L2
 // each) receives a contiguous band of tiles rather than every 8th tile.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -11,6 +12,7 @@
 #include "rt_internal.h"
 #include "svo_device.h"
 #include "trace_common.h"
+#include "render_pixel.h"
 
 using namespace rtdev;
 
@@ -63,42 +65,6 @@ __device__ __forceinline__ void TilePixel(uint32_t tile, uint32_t tiles_x, int& 
   y = int((tile / tiles_x) * 16u + (wave >> 1) * 8u + (lane >> 3));
 }
 
-__device__ __forceinline__ uint32_t LoadU32(const uint32_t* p, int64_t i, uint32_t fallback) {
-  return p ? p[i] : fallback;
-}
-
-// Colour of ProcessHit for `node`, optionally with light 1 occluded.
-__device__ __forceinline__ uint32_t ShadeNode(const uint32_t* normals, const uint32_t* colors, int node,
-                                              const Lights& lt, bool occluded) {
-  const uint32_t n = LoadU32(normals, node, 0u);
-  const uint32_t c = LoadU32(colors, node, 0xffffffffu);
-  return ShadeColor(n, c, lt, occluded);
-}
-
-// Shadow-ray test for a primary hit (extension; no reference counterpart).
-// Cast from just before the hit point (a quarter of the hit voxel back along
-// the primary ray, which is empty space) towards light 1, without LOD.
-template <class Stack, bool COUNT>
-__device__ __forceinline__ bool ShadowOccluded(const uint32_t* links, const uint32_t* far_ptrs, uint32_t root,
-                                               float ox, float oy, float oz, float dx, float dy, float dz,
-                                               const CastResult& hit, const Lights& lt, Stack& stack,
-                                               uint32_t& fetches, uint32_t& shadow_rays) {
-  const int nd = hit.node_depth;
-  const float eps = __uint_as_float(uint32_t(127 - nd - 2) << 23);
-  const float tb = cl_max(0.0f, hit.hit_t - eps);
-  const float sx = ox + dx * tb, sy = oy + dy * tb, sz = oz + dz * tb;
-  CastResult s;
-  CastRay<Stack, COUNT>(links, far_ptrs, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
-                        0.0f, 0.0f, stack, s, fetches);
-  if (COUNT) ++shadow_rays;
-  return s.error_code == 0 && s.fault_block == -1 && s.hit_node != -1;
-}
-
-__device__ __forceinline__ int HitCode(int hit_node, int dup, int nodes_in_block) {
-  const int unit = nodes_in_block > 0 ? hit_node / nodes_in_block : hit_node;
-  return ((unit + 1) << 1) | dup;
-}
-
 // ------------------------------------------------------------ InitTracingFrame
 __global__ __launch_bounds__(256) void InitTracingFrameKernel(float4* __restrict__ org, float4* __restrict__ dir,
                                                               int4* __restrict__ misc, int W, int H, Mat16 m,
@@ -118,8 +84,7 @@ __global__ __launch_bounds__(256) void InitTracingFrameKernel(float4* __restrict
 template <int LDS_LEVELS>
 __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ codes, float4* __restrict__ org,
                                                             const float4* __restrict__ dir, int4* __restrict__ misc,
-                                                            int W, int H, const uint32_t* __restrict__ links,
-                                                            const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                            int W, int H, PoolView pool, uint32_t root,
                                                             const uint32_t* __restrict__ normals,
                                                             const uint32_t* __restrict__ colors, int nodes_in_block,
                                                             int flags, Lights lt) {
@@ -140,7 +105,7 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
   const float4 d = dir[i];
   CastResult res;
   uint32_t fetches = 0, shadow_rays = 0;
-  CastRay<SplitStack<LDS_LEVELS, 256>, false>(links, far_ptrs, root, o.x, o.y, o.z, d.x, d.y, d.z, d.w, o.w, stack,
+  CastRay<SplitStack<LDS_LEVELS, 256>, false>(pool, root, o.x, o.y, o.z, d.x, d.y, d.z, d.w, o.w, stack,
                                               res, fetches);
   int code;
   if (res.error_code) {
@@ -162,7 +127,7 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
       const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
                   nz = float(int8_t((n >> 16) & 255u)) / 127.f;
       if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
-        occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, false>(links, far_ptrs, root, o.x, o.y, o.z, d.x, d.y,
+        occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, false>(pool, root, o.x, o.y, o.z, d.x, d.y,
                                                                     d.z, res, lt, stack, fetches, shadow_rays);
     }
     s.y = int(ShadeNode(normals, colors, res.hit_node, lt, occluded));
@@ -192,8 +157,7 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 // ---------------------------------------------------------------- plain fused
 template <int LDS_LEVELS, bool COUNT>
 __global__ __launch_bounds__(256) void RenderPlainKernel(uint32_t* __restrict__ out, int* __restrict__ codes, int W,
-                                                         int H, const uint32_t* __restrict__ links,
-                                                         const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                         int H, PoolView pool, uint32_t root,
                                                          const uint32_t* __restrict__ normals,
                                                          const uint32_t* __restrict__ colors, int nodes_in_block,
                                                          Mat16 m, float lod, Vec4 bg, int flags, Lights lt,
@@ -202,64 +166,24 @@ __global__ __launch_bounds__(256) void RenderPlainKernel(uint32_t* __restrict__ 
   const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
   int x, y;
   TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
-  uint32_t fetches = 0, shadow_rays = 0, hits = 0, primaries = 0;
+  PixelCounters cnt;
   if (x < W && y < H) {
     const int64_t i = int64_t(y) * W + x;
     SplitStack<LDS_LEVELS, 256> stack;
     stack.lds = lds + threadIdx.x;
-    const PrimaryRay r = InitRay(m.m, x, y, W, H, lod);
-    CastResult res;
-    CastRay<SplitStack<LDS_LEVELS, 256>, COUNT>(links, far_ptrs, root, r.origin.x, r.origin.y, r.origin.z,
-                                                r.direction.x, r.direction.y, r.direction.z, r.direction.w,
-                                                r.origin.w, stack, res, fetches);
-    if (COUNT) ++primaries;
     uint32_t color;
     int code;
-    if (res.error_code) {
-      color = PackRGBA(255, 0, 0, 255);
-      code = kNoHit;
-    } else if (res.fault_block != -1) {
-      // Unfinished after the single pass: FinishTracingFrame shades the fault parent.
-      color = ShadeNode(normals, colors, res.hit_node, lt, false);
-      code = -res.fault_block - 1;
-    } else if (res.hit_node == -1) {
-      color = 0u;
-      code = kNoHit;
-    } else {
-      bool occluded = false;
-      if ((flags & RT_FLAG_SHADOWS) && normals) {
-        const uint32_t n = normals[res.hit_node];
-        const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
-                    nz = float(int8_t((n >> 16) & 255u)) / 127.f;
-        if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
-          occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, COUNT>(
-              links, far_ptrs, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
-              r.direction.z, res, lt, stack, fetches, shadow_rays);
-      }
-      color = ShadeNode(normals, colors, res.hit_node, lt, occluded);
-      code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);
-      if (COUNT) ++hits;
-    }
-    out[i] = Blend(color, bg.v);
+    RenderPixel<SplitStack<LDS_LEVELS, 256>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y, W, H,
+                                                    lod, bg.v, flags, lt, stack, color, code, cnt);
+    out[i] = color;
     if (codes) codes[i] = code;
   }
-  if (COUNT) {
-    __shared__ unsigned long long acc[4];
-    if (threadIdx.x < 4) acc[threadIdx.x] = 0;
-    __syncthreads();
-    atomicAdd(&acc[0], (unsigned long long)primaries);
-    atomicAdd(&acc[1], (unsigned long long)shadow_rays);
-    atomicAdd(&acc[2], (unsigned long long)fetches);
-    atomicAdd(&acc[3], (unsigned long long)hits);
-    __syncthreads();
-    if (threadIdx.x < 4) atomicAdd(&stats[threadIdx.x], acc[threadIdx.x]);
-  }
+  if (COUNT) FlushCounters(cnt, stats);
 }
 
 // ----------------------------------------------------- RaytraceKernel (depth)
 __global__ __launch_bounds__(256) void RaytraceDepthKernel(uint32_t* __restrict__ out, int W, int H,
-                                                           const uint32_t* __restrict__ links,
-                                                           const uint32_t* __restrict__ far_ptrs, uint32_t root,
+                                                           PoolView pool, uint32_t root,
                                                            Mat16 m, float lod) {
   __shared__ uint2 lds[kStackLevels * 256];
   const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
@@ -287,7 +211,7 @@ __global__ __launch_bounds__(256) void RaytraceDepthKernel(uint32_t* __restrict_
   stack.lds = lds + threadIdx.x;
   CastResult res;
   uint32_t fetches = 0;
-  CastRay<SplitStack<kStackLevels, 256>, false>(links, far_ptrs, root, origin.x, origin.y, origin.z, d.x, d.y, d.z,
+  CastRay<SplitStack<kStackLevels, 256>, false>(pool, root, origin.x, origin.y, origin.z, d.x, d.y, d.z,
                                                 coef, bias, stack, res, fetches);
   const int64_t i = int64_t(y) * W + x;
   if (res.error_code || res.fault_block != -1) {
@@ -372,30 +296,39 @@ uint32_t TileGrid(int W, int H) { return uint32_t((W + 15) / 16) * uint32_t((H +
 
 const uint32_t* U32(const rt_buf* b) { return b ? static_cast<const uint32_t*>(b->ptr) : nullptr; }
 
+PoolView View(const rt_buf* links, const rt_buf* far_ptrs) {
+  PoolView v;
+  v.links = U32(links);
+  v.far_ptrs = U32(far_ptrs);
+  v.nodes = uint32_t(std::min<uint64_t>(links->size / 4, 0xffffffffu));
+  v.far_count = uint32_t(std::min<uint64_t>(far_ptrs->size / 4, 0xffffffffu));
+  return v;
+}
+
 template <int L>
 void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc, int W, int H,
-                 const uint32_t* links, const uint32_t* fp, uint32_t root, const uint32_t* nrm, const uint32_t* col,
+                 const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col,
                  int nib, int flags, const Lights& lt) {
-  RaytracingPassKernel<L><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, links, fp, root, nrm, col, nib, flags,
+  RaytracingPassKernel<L><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags,
                                                   lt);
 }
 
 template <int L, bool C>
-void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const uint32_t* links,
-                  const uint32_t* fp, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
+void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
+                  uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
                   const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, unsigned long long* st) {
-  RenderPlainKernel<L, C><<<g, dim3(256), 0, s>>>(out, codes, W, H, links, fp, root, nrm, col, nib, m, lod, bg,
+  RenderPlainKernel<L, C><<<g, dim3(256), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg,
                                                   flags, lt, st);
 }
 
 template <bool C>
 bool DispatchPlain(int levels, dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H,
-                   const uint32_t* links, const uint32_t* fp, uint32_t root, const uint32_t* nrm,
+                   const PoolView& pool, uint32_t root, const uint32_t* nrm,
                    const uint32_t* col, int nib, const Mat16& m, float lod, const Vec4& bg, int flags,
                    const Lights& lt, unsigned long long* st) {
   switch (levels) {
 #define RT_CASE(L) \
-  case L: LaunchPlainL<L, C>(g, s, out, codes, W, H, links, fp, root, nrm, col, nib, m, lod, bg, flags, lt, st); return true;
+  case L: LaunchPlainL<L, C>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, st); return true;
     RT_CASE(8) RT_CASE(10) RT_CASE(12) RT_CASE(14) RT_CASE(16) RT_CASE(22)
 #undef RT_CASE
     default:
@@ -440,7 +373,7 @@ int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int
   const Lights lt = rayt::MakeLights();
   LaunchPassL<kStackLevels>(dim3(TileGrid(width, height)), ctx->stream, static_cast<int*>(faults_and_hits->ptr),
                             reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + 16 * P),
-                            reinterpret_cast<int4*>(base + 32 * P), width, height, U32(node_links), U32(far_pointers),
+                            reinterpret_cast<int4*>(base + 32 * P), width, height, View(node_links, far_pointers),
                             uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, flags, lt);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
@@ -475,7 +408,7 @@ int rt_raytrace_depth(rt_ctx* ctx, rt_buf* out_image, int width, int height, con
   Mat16 m;
   memcpy(m.m, vpi, sizeof(m.m));
   RaytraceDepthKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
-      static_cast<uint32_t*>(out_image->ptr), width, height, U32(node_links), U32(far_pointers),
+      static_cast<uint32_t*>(out_image->ptr), width, height, View(node_links, far_pointers),
       uint32_t(root_node_index), m, lod_voxel_size);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
@@ -552,8 +485,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
     RT_HIP_CHECK(hipMalloc(&d_stats, 4 * sizeof(unsigned long long)));
     hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream);
     DispatchPlain<true>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
-                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, U32(node_links),
-                        U32(far_pointers), uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
+                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
+                        uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
                         lod_voxel_size, bg, flags, lt, d_stats);
     hipError_t e = hipGetLastError();
     unsigned long long h[4] = {0, 0, 0, 0};
@@ -565,8 +498,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
     return RT_OK;
   }
   DispatchPlain<false>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
-                       codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, U32(node_links),
-                       U32(far_pointers), uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
+                       codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
+                       uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
                        lod_voxel_size, bg, flags, lt, nullptr);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;

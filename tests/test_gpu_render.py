@@ -35,7 +35,7 @@ def _oracle_arrays(t, c):
     return links, far, normals, colors
 
 
-@pytest.mark.parametrize("level,nib", [(3, 131), (5, 111), (7, 111), (6, 8)])
+@pytest.mark.parametrize("level,nib", [(3, 131), (5, 111), (7, 111), (6, 16)])
 def test_sphere_pool_layout(gpu_ctx, level, nib):
     """CheckTestOctreeSphereWithGPUData on the device pool after InitialFillCache (whole tree)."""
     center, radius = (0.3, 0.4, 0.65), 0.23
