@@ -29,6 +29,83 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+MIX_SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+MIX_WEIGHTS = [0.3, 0.3, 0.2, 0.15, 0.05]
+
+
+def view_mix(n: int, seed: int):
+    """n views: seeded resolutions from the mix, positions uniform in [0,1]^3, directions uniform on the sphere."""
+    import math
+    import numpy as np
+    from rayt_amd import api
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        w, h = MIX_SIZES[rng.choice(len(MIX_SIZES), p=MIX_WEIGHTS)]
+        pos = tuple(float(x) for x in rng.uniform(0.0, 1.0, 3))
+        yaw = float(rng.uniform(-180.0, 180.0))
+        pitch = math.degrees(math.asin(float(rng.uniform(-1.0, 1.0))))
+        vpi = api.Camera(yaw=yaw, pitch=pitch, position=pos, fov=60.0, aspect=w / h).view_proj_inv()
+        specs.append((vpi, w, h))
+    return specs
+
+
+def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: int = 7):
+    """64-view mix in one persistent launch vs the plain kernel launched once per view."""
+    import numpy as np
+    from rayt_amd import api
+    flags = api.L.RT_FLAG_SHADOWS
+    specs = view_mix(nviews, seed)
+    b = api.BatchRenderer(cache, nviews)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=4, tree_depth=depth)
+    views, total, offsets = b.make_views(specs, flags=flags)
+    out = api.Buffer(ctx, 4 * total)
+    codes = api.Buffer(ctx, 4 * total)
+    st = b.render(views, nviews, out, codes, count=True)
+    rays = st[0] + st[1]
+    t = time.time()
+    best, best_ms = b.autotune(views, nviews, out, codes, repeats=2)
+    log(f"batched: {nviews} views, {total} pixels, {rays} rays; autotune {time.time() - t:.1f}s -> "
+        f"tile {api.BatchRenderer.TILE_SHAPES[best[0]]}, {best[1]} LDS levels, {best[2]} WG/CU ({best_ms:.2f} ms)")
+    ms = []
+    for _ in range(steps):
+        b.render(views, nviews, out, codes)
+        ms.append(b.last_ms())
+    batched_ms = float(np.median(ms))
+    img = out.read_all(np.uint32)
+    # the plain kernel, one launch per view, same views
+    pout = api.Buffer(ctx, 4 * 3840 * 2160)
+    pcodes = api.Buffer(ctx, 4 * 3840 * 2160)
+    ctx.WaitForAll()
+    reps = max(1, steps // 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for vpi, w, h in specs:
+            api.render_plain(ctx, cache, w, h, vpi, lod=0.0, flags=flags, out=pout, codes=pcodes)
+    ctx.WaitForAll()
+    plain_ms = (time.perf_counter() - t0) / reps * 1e3
+    mismatches = 0
+    checked = list(range(0, nviews, max(1, nviews // 8)))
+    for k in checked:
+        vpi, w, h = specs[k]
+        api.render_plain(ctx, cache, w, h, vpi, lod=0.0, flags=flags, out=pout, codes=pcodes)
+        if not np.array_equal(pout.Read(0, 4 * w * h, np.uint32), img[offsets[k]:offsets[k] + w * h]):
+            mismatches += 1
+    res = {"views": nviews, "pixels": total, "rays": rays, "ms": round(batched_ms, 3),
+           "mrays_s": round(rays / batched_ms / 1e3, 1), "plain_per_view_ms": round(plain_ms, 3),
+           "plain_per_view_mrays_s": round(rays / plain_ms / 1e3, 1),
+           "speedup_vs_plain": round(plain_ms / batched_ms, 3),
+           "config": {"tile": "x".join(map(str, api.BatchRenderer.TILE_SHAPES[best[0]])), "lds_levels": best[1],
+                      "wgs_per_cu": best[2]},
+           "views_checked_bit_exact": len(checked) - mismatches}
+    log(f"batched {res['mrays_s']} Mrays/s ({batched_ms:.2f} ms) vs plain per-view launches {res['plain_per_view_mrays_s']} "
+        f"Mrays/s ({plain_ms:.2f} ms): {res['speedup_vs_plain']}x; view mismatches {mismatches}")
+    if mismatches:
+        raise SystemExit("batched output differs from the plain kernel")
+    b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,6 +116,9 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="also run a device-to-device copy of MB MiB (3 times) as the byte-counter reference")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -56,6 +136,14 @@ def main():
     log(f"scene depth {args.depth}: {info.source_nodes} nodes, {info.blocks_count} blocks, built in {time.time() - t:.1f}s")
 
     ctx = api.Context(0)
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        src, dst = api.Buffer(ctx, n), api.Buffer(ctx, n)
+        for _ in range(3):
+            dst.CopyFrom(src)
+        ctx.WaitForAll()
+        log(f"calibration copy: 3 x {n} bytes")
+        del src, dst
     t = time.time()
     cache = api.GPUOctreeCacheManager(info.blocks_count, tree, ctx)
     cache.InitialFillCache()
@@ -97,6 +185,7 @@ def main():
         f"({rays / avg_kernel_ms / 1e3:.0f} Mrays/s), {achieved_gbs:.0f} GB/s algorithmic")
 
     img = tr.out_image()
+    batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3))
     import oracle as orc  # CPU baseline + one-frame check of the measured output
 
     cpu = None
@@ -145,6 +234,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "kernel_mrays_s": round(rays / avg_kernel_ms / 1e3, 1),
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -39,6 +39,12 @@ struct Vec4B {
   float v[4];
 };
 
+// Lanes that need a ray pull the next pixel of the wave's chunk; a chunk is
+// kTilesPerGrab wave tiles claimed from the XCD queues. Refills happen when
+// at least kRefillMin lanes are idle, so the cost of the bookkeeping is paid
+// once per several traversal steps.
+constexpr int kRefillMin = 16;
+
 template <int TW, int LDS_LEVELS, bool COUNT>
 __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ views, int nviews,
                                                    const uint32_t* __restrict__ tile_begin, uint32_t total_tiles,
@@ -55,45 +61,134 @@ __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ v
   SplitStack<LDS_LEVELS, 256> stack;
   stack.lds = lds + threadIdx.x;
   stack.base = lds_base;
-  const int lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   const uint32_t xcd = blockIdx.x & 7u;
   const uint32_t per = (total_tiles + 7u) / 8u;
   PixelCounters cnt;
-  for (uint32_t k = 0; k < 8; ++k) {
-    const uint32_t q = (xcd + k) & 7u;
-    const uint32_t qbegin = min(total_tiles, q * per), qend = min(total_tiles, qbegin + per);
-    while (true) {
-      uint32_t t = 0;
-      if (lane == 0) t = atomicAdd(&counters[q], kTilesPerGrab);
-      t = __shfl(t, 0, 64);
-      const uint32_t first = qbegin + t;
-      if (t >= per || first >= qend) break;
-      const uint32_t last = min(qend, first + kTilesPerGrab);
-      for (uint32_t tile = first; tile < last; ++tile) {
-        // view holding this tile: largest v with begin_s[v] <= tile
-        int lo = 0, hi = nviews - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (begin_s[mid] <= tile) lo = mid; else hi = mid - 1;
+  uint32_t fetches = 0;
+
+  // wave-uniform chunk state
+  uint32_t q_rel = 0, chunk_first = 0, chunk_pix = 0, chunk_npix = 0;
+  bool queues_done = false;
+
+  // lane state
+  bool active = false;
+  int phase = 0;  // 0: primary ray, 1: shadow ray
+  Trav tr;
+  uint64_t pix = 0;
+  int flags = 0, code = 0, hit_node = 0;
+  float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
+
+  while (true) {
+    uint64_t need = __ballot(!active);
+    if (!queues_done && __popcll(need) >= kRefillMin) {
+      while (need && !queues_done) {
+        if (chunk_pix >= chunk_npix) {
+          while (true) {  // claim the next chunk from this XCD's queue, then steal from the others
+            if (q_rel >= 8) {
+              queues_done = true;
+              break;
+            }
+            const uint32_t q = (xcd + q_rel) & 7u;
+            const uint32_t qbegin = min(total_tiles, q * per), qend = min(total_tiles, qbegin + per);
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(&counters[q], kTilesPerGrab);
+            t = __shfl(t, 0, 64);
+            if (t < per && qbegin + t < qend) {
+              chunk_first = qbegin + t;
+              chunk_npix = (min(qend, chunk_first + kTilesPerGrab) - chunk_first) * 64u;
+              chunk_pix = 0;
+              break;
+            }
+            ++q_rel;
+          }
+          if (queues_done) break;
         }
-        const DevView& v = views[lo];
-        const uint32_t local = tile - begin_s[lo];
-        const int x = int(local % uint32_t(v.tiles_x)) * TW + (lane % TW);
-        const int y = int(local / uint32_t(v.tiles_x)) * TH + (lane / TW);
-        if (x < v.width && y < v.height) {
-          uint32_t color;
-          int code;
-          RenderPixel<SplitStack<LDS_LEVELS, 256>, COUNT>(pool, root, normals, colors, nodes_in_block, v.m, x, y,
-                                                          v.width, v.height, v.lod, bg.v, v.flags, lt, stack, color,
-                                                          code, cnt);
-          const uint64_t i = v.pixel_offset + uint64_t(y) * v.width + x;
-          out[i] = color;
-          if (codes) codes[i] = code;
+        const uint32_t avail = chunk_npix - chunk_pix;
+        const uint32_t rank = uint32_t(__popcll(need & lanes_below));
+        const uint32_t take = min(uint32_t(__popcll(need)), avail);
+        if (!active && rank < take) {
+          const uint32_t p = chunk_pix + rank;
+          const uint32_t tile = chunk_first + p / 64u, local_px = p % 64u;
+          int lo = 0, hi = nviews - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (begin_s[mid] <= tile) lo = mid; else hi = mid - 1;
+          }
+          const DevView& v = views[lo];
+          const uint32_t local = tile - begin_s[lo];
+          const int x = int(local % uint32_t(v.tiles_x)) * TW + int(local_px % TW);
+          const int y = int(local / uint32_t(v.tiles_x)) * TH + int(local_px / TW);
+          if (x < v.width && y < v.height) {
+            const PrimaryRay r = InitRay(v.m, x, y, v.width, v.height, v.lod);
+            ox = r.origin.x, oy = r.origin.y, oz = r.origin.z;
+            dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
+            TravInit(tr, root, ox, oy, oz, dx, dy, dz, r.direction.w, r.origin.w);
+            pix = v.pixel_offset + uint64_t(y) * v.width + x;
+            flags = v.flags;
+            phase = 0;
+            active = true;
+            if (COUNT) ++cnt.primaries;
+          }
         }
+        chunk_pix += take;
+        need = __ballot(!active);
       }
     }
+    if (!__any(active)) {
+      if (queues_done) break;
+      continue;  // every claimed pixel so far was outside its view; claim more
+    }
+    if (!active) continue;
+    CastResult res;
+    if (!TravStep<SplitStack<LDS_LEVELS, 256>, COUNT>(tr, pool, stack, res, fetches)) continue;
+    uint32_t color;
+    if (phase == 0) {
+      if (res.error_code) {
+        color = PackRGBA(255, 0, 0, 255);
+        code = kNoHit;
+      } else if (res.fault_block != -1) {
+        color = ShadeNode(normals, colors, res.hit_node, lt, false);
+        code = -res.fault_block - 1;
+      } else if (res.hit_node == -1) {
+        color = 0u;
+        code = kNoHit;
+      } else {
+        code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);
+        hit_node = res.hit_node;
+        if (COUNT) ++cnt.hits;
+        bool start_shadow = false;
+        if ((flags & 1) && normals) {
+          const uint32_t n = normals[res.hit_node];
+          const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
+                      nz = float(int8_t((n >> 16) & 255u)) / 127.f;
+          start_shadow = cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f;
+        }
+        if (start_shadow) {
+          // same origin rule as ShadowOccluded
+          const float eps = __uint_as_float(uint32_t(127 - res.node_depth - 2) << 23);
+          const float tb = cl_max(0.0f, res.hit_t - eps);
+          TravInit(tr, root, ox + dx * tb, oy + dy * tb, oz + dz * tb, lt.shadow_dir[0], lt.shadow_dir[1],
+                   lt.shadow_dir[2], 0.0f, 0.0f);
+          phase = 1;
+          if (COUNT) ++cnt.shadow_rays;
+          continue;
+        }
+        color = ShadeNode(normals, colors, hit_node, lt, false);
+      }
+    } else {
+      const bool occluded = res.error_code == 0 && res.fault_block == -1 && res.hit_node != -1;
+      color = ShadeNode(normals, colors, hit_node, lt, occluded);
+    }
+    out[pix] = Blend(color, bg.v);
+    if (codes) codes[pix] = code;
+    active = false;
   }
-  if (COUNT) FlushCounters(cnt, stats);
+  if (COUNT) {
+    cnt.fetches = fetches;
+    FlushCounters(cnt, stats);
+  }
 }
 
 __global__ void ZeroCountersKernel(uint32_t* c, int n) {

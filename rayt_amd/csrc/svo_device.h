@@ -87,171 +87,204 @@ struct PoolView {
   uint32_t far_count;
 };
 
-// ESVO traversal over the cache link format. `fetches` counts executed
-// node-link and far-pointer loads (the F of the roofline accounting).
-template <class Stack, bool COUNT>
-__device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node,
-                                        float px, float py, float pz, float dx, float dy, float dz,
-                                        float ray_size_coef, float ray_size_bias, Stack& stack,
-                                        CastResult& res, uint32_t& fetches) {
-  const uint32_t* __restrict__ node_links = pool.links;
-  const uint32_t* __restrict__ far_pointers = pool.far_ptrs;
+// ESVO traversal over the cache link format, as a resumable state machine:
+// TravInit is the set-up before the loop of raycast.cl:42-95 and TravStep is
+// one iteration of the loop of raycast.cl:100-250 (including the loop test).
+// CastRay runs the steps to the end; the batched kernel interleaves steps of
+// different rays in one lane. `fetches` counts executed node-link and
+// far-pointer loads (the F of the roofline accounting).
+struct Trav {
+  float tx_coef, ty_coef, tz_coef, tx_bias, ty_bias, tz_bias;
+  float ray_size_coef, ray_size_bias;
+  float t_min, t_max, h;
+  float posx, posy, posz, scale_exp2;
+  uint32_t parent, child_descriptor, idx, octant_mask;
+  int scale, iter;
+};
+
+__device__ __forceinline__ void TravInit(Trav& s, uint32_t root_node, float px, float py, float pz, float dx, float dy,
+                                         float dz, float ray_size_coef, float ray_size_bias) {
   px += 1.0f;
   py += 1.0f;
   pz += 1.0f;
-  res.fault_block = -1;
-  res.error_code = 0;
-  res.hit_duplicate = 0;
-
   if (fabsf(dx) < kEpsilon) dx = copysignf(kEpsilon, dx);
   if (fabsf(dy) < kEpsilon) dy = copysignf(kEpsilon, dy);
   if (fabsf(dz) < kEpsilon) dz = copysignf(kEpsilon, dz);
+  s.tx_coef = 1.0f / -fabsf(dx);
+  s.ty_coef = 1.0f / -fabsf(dy);
+  s.tz_coef = 1.0f / -fabsf(dz);
+  s.tx_bias = s.tx_coef * px;
+  s.ty_bias = s.ty_coef * py;
+  s.tz_bias = s.tz_coef * pz;
+  s.octant_mask = 0;
+  if (dx > 0.0f) s.octant_mask ^= 1u, s.tx_bias = 3.0f * s.tx_coef - s.tx_bias;
+  if (dy > 0.0f) s.octant_mask ^= 2u, s.ty_bias = 3.0f * s.ty_coef - s.ty_bias;
+  if (dz > 0.0f) s.octant_mask ^= 4u, s.tz_bias = 3.0f * s.tz_coef - s.tz_bias;
+  s.t_min = cl_fmax(cl_fmax(2.0f * s.tx_coef - s.tx_bias, 2.0f * s.ty_coef - s.ty_bias), 2.0f * s.tz_coef - s.tz_bias);
+  s.t_max = cl_fmin(cl_fmin(s.tx_coef - s.tx_bias, s.ty_coef - s.ty_bias), s.tz_coef - s.tz_bias);
+  s.h = s.t_max;
+  s.t_min = cl_fmax(s.t_min, 0.0f);
+  s.parent = root_node;
+  s.child_descriptor = 0;
+  s.idx = 0;
+  s.posx = s.posy = s.posz = 1.0f;
+  s.scale = kSMax - 1;
+  s.scale_exp2 = 0.5f;
+  if (1.5f * s.tx_coef - s.tx_bias > s.t_min) s.idx ^= 1u, s.posx = 1.5f;
+  if (1.5f * s.ty_coef - s.ty_bias > s.t_min) s.idx ^= 2u, s.posy = 1.5f;
+  if (1.5f * s.tz_coef - s.tz_bias > s.t_min) s.idx ^= 4u, s.posz = 1.5f;
+  s.iter = 0;
+  s.ray_size_coef = ray_size_coef;
+  s.ray_size_bias = ray_size_bias;
+}
 
-  const float tx_coef = 1.0f / -fabsf(dx);
-  const float ty_coef = 1.0f / -fabsf(dy);
-  const float tz_coef = 1.0f / -fabsf(dz);
-  float tx_bias = tx_coef * px;
-  float ty_bias = ty_coef * py;
-  float tz_bias = tz_coef * pz;
-
-  uint32_t octant_mask = 0;
-  if (dx > 0.0f) octant_mask ^= 1u, tx_bias = 3.0f * tx_coef - tx_bias;
-  if (dy > 0.0f) octant_mask ^= 2u, ty_bias = 3.0f * ty_coef - ty_bias;
-  if (dz > 0.0f) octant_mask ^= 4u, tz_bias = 3.0f * tz_coef - tz_bias;
-
-  float t_min = cl_fmax(cl_fmax(2.0f * tx_coef - tx_bias, 2.0f * ty_coef - ty_bias), 2.0f * tz_coef - tz_bias);
-  float t_max = cl_fmin(cl_fmin(tx_coef - tx_bias, ty_coef - ty_bias), tz_coef - tz_bias);
-  float h = t_max;
-  t_min = cl_fmax(t_min, 0.0f);
-
-  uint32_t parent = root_node;
-  uint32_t child_descriptor = 0;
-  uint32_t idx = 0;
-  float posx = 1.0f, posy = 1.0f, posz = 1.0f;
-  int scale = kSMax - 1;
-  float scale_exp2 = 0.5f;
-
-  if (1.5f * tx_coef - tx_bias > t_min) idx ^= 1u, posx = 1.5f;
-  if (1.5f * ty_coef - ty_bias > t_min) idx ^= 2u, posy = 1.5f;
-  if (1.5f * tz_coef - tz_bias > t_min) idx ^= 4u, posz = 1.5f;
-
-  int iter = 0;
-  while (scale < kSMax) {
-    if (++iter > kIterLimit) {
-      res.error_code = 1;
-      res.hit_t = t_min;
-      res.hit_node = int(parent);
-      res.node_depth = kSMax - scale - 1;
-      return;
-    }
-    if (child_descriptor == 0) {
-      child_descriptor = node_links[parent];
-      if (COUNT) ++fetches;
-    }
-    const float tx_corner = posx * tx_coef - tx_bias;
-    const float ty_corner = posy * ty_coef - ty_bias;
-    const float tz_corner = posz * tz_coef - tz_bias;
-    const float tc_max = cl_fmin(cl_fmin(tx_corner, ty_corner), tz_corner);
-
-    const uint32_t child_shift = idx ^ octant_mask;
-    const uint32_t child_masks = 1u << child_shift;
-    if ((child_descriptor & child_masks) != 0 && t_min <= t_max) {
-      if (child_descriptor & (1u << 8)) {
-        res.hit_node = int(parent);
-        res.hit_t = t_min;
-        res.fault_block = int(child_descriptor >> 11);
-        res.node_depth = kSMax - scale - 1;
-        return;
-      }
-      uint32_t ofs;
-      if (child_descriptor & (1u << 10)) {
-        if ((child_descriptor >> 11) >= pool.far_count) {
-          res.error_code = 2;
-          res.hit_t = t_min;
-          res.hit_node = int(parent);
-          res.node_depth = kSMax - scale - 1;
-          return;
-        }
-        ofs = far_pointers[child_descriptor >> 11];
-        if (COUNT) ++fetches;
-      } else {
-        ofs = (child_descriptor >> 11) + parent - (1u << 20);
-      }
-      ofs += __popc(child_descriptor & (child_masks - 1u) & 255u);
-      if (ofs >= pool.nodes || scale < 1) {
-        res.error_code = 2;
-        res.hit_t = t_min;
-        res.hit_node = int(parent);
-        res.node_depth = kSMax - scale - 1;
-        return;
-      }
-      child_descriptor = node_links[ofs];
-      if (COUNT) ++fetches;
-
-      if (!(child_descriptor & 255u) || tc_max * ray_size_coef + ray_size_bias >= scale_exp2) {
-        res.hit_t = t_min;
-        res.hit_node = int(ofs);
-        res.hit_duplicate = int((child_descriptor >> 9) & 1u);
-        res.node_depth = kSMax - scale;
-        return;
-      }
-
-      const float half = scale_exp2 * 0.5f;
-      const float tx_center = half * tx_coef + tx_corner;
-      const float ty_center = half * ty_coef + ty_corner;
-      const float tz_center = half * tz_coef + tz_corner;
-
-      if (tc_max < h) stack.put(scale, make_uint2(parent, __float_as_uint(t_max)));
-      h = tc_max;
-      parent = ofs;
-
-      idx = 0;
-      scale--;
-      scale_exp2 = half;
-      if (tx_center > t_min) idx ^= 1u, posx += scale_exp2;
-      if (ty_center > t_min) idx ^= 2u, posy += scale_exp2;
-      if (tz_center > t_min) idx ^= 4u, posz += scale_exp2;
-      t_max = tc_max;
-      continue;
-    }
-
-    uint32_t step_mask = 0;
-    if (tx_corner <= tc_max) step_mask ^= 1u, posx -= scale_exp2;
-    if (ty_corner <= tc_max) step_mask ^= 2u, posy -= scale_exp2;
-    if (tz_corner <= tc_max) step_mask ^= 4u, posz -= scale_exp2;
-    t_min = tc_max;
-    idx ^= step_mask;
-
-    if ((idx & step_mask) != 0) {
-      uint32_t differing_bits = 0;
-      if (step_mask & 1u) differing_bits |= __float_as_uint(posx) ^ __float_as_uint(posx + scale_exp2);
-      if (step_mask & 2u) differing_bits |= __float_as_uint(posy) ^ __float_as_uint(posy + scale_exp2);
-      if (step_mask & 4u) differing_bits |= __float_as_uint(posz) ^ __float_as_uint(posz + scale_exp2);
-      // Highest set bit. The reference reads it from the exponent of
-      // (float)differing_bits; the two agree whenever the result is below
-      // S_MAX (the value is then exact in fp32), and both leave the loop
-      // otherwise.
-      scale = 31 - __clz(differing_bits);
-      if (scale >= kSMax) break;  // left the root cube
-      scale_exp2 = __uint_as_float(uint32_t(scale - kSMax + 127) << 23);
-
-      const uint2 e = stack.get(scale);
-      parent = e.x;
-      t_max = __uint_as_float(e.y);
-
-      const uint32_t shx = __float_as_uint(posx) >> scale;
-      const uint32_t shy = __float_as_uint(posy) >> scale;
-      const uint32_t shz = __float_as_uint(posz) >> scale;
-      posx = __uint_as_float(shx << scale);
-      posy = __uint_as_float(shy << scale);
-      posz = __uint_as_float(shz << scale);
-      idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
-      h = 0.0f;
-      child_descriptor = 0;
-    }
+// One loop iteration. Returns true when the ray has finished; `res` then holds
+// hit_t, hit_node, fault_block, hit_duplicate, error_code and node_depth.
+template <class Stack, bool COUNT>
+__device__ __forceinline__ bool TravStep(Trav& s, const PoolView& pool, Stack& stack, CastResult& res,
+                                         uint32_t& fetches) {
+  if (s.scale >= kSMax) {  // left the octree: miss
+    res.fault_block = -1;
+    res.error_code = 0;
+    res.hit_duplicate = 0;
+    res.hit_t = 1e30f;
+    res.hit_node = -1;
+    res.node_depth = 0;
+    return true;
   }
-  res.hit_t = 1e30f;
-  res.hit_node = -1;
+  if (++s.iter > kIterLimit) {
+    res.fault_block = -1;
+    res.hit_duplicate = 0;
+    res.error_code = 1;
+    res.hit_t = s.t_min;
+    res.hit_node = int(s.parent);
+    res.node_depth = kSMax - s.scale - 1;
+    return true;
+  }
+  if (s.child_descriptor == 0) {
+    s.child_descriptor = pool.links[s.parent];
+    if (COUNT) ++fetches;
+  }
+  const float tx_corner = s.posx * s.tx_coef - s.tx_bias;
+  const float ty_corner = s.posy * s.ty_coef - s.ty_bias;
+  const float tz_corner = s.posz * s.tz_coef - s.tz_bias;
+  const float tc_max = cl_fmin(cl_fmin(tx_corner, ty_corner), tz_corner);
+
+  const uint32_t child_shift = s.idx ^ s.octant_mask;
+  const uint32_t child_masks = 1u << child_shift;
+  if ((s.child_descriptor & child_masks) != 0 && s.t_min <= s.t_max) {
+    if (s.child_descriptor & (1u << 8)) {  // children not loaded: fault
+      res.hit_duplicate = 0;
+      res.error_code = 0;
+      res.hit_node = int(s.parent);
+      res.hit_t = s.t_min;
+      res.fault_block = int(s.child_descriptor >> 11);
+      res.node_depth = kSMax - s.scale - 1;
+      return true;
+    }
+    uint32_t ofs;
+    if (s.child_descriptor & (1u << 10)) {
+      if ((s.child_descriptor >> 11) >= pool.far_count) {
+        res.fault_block = -1;
+        res.hit_duplicate = 0;
+        res.error_code = 2;
+        res.hit_t = s.t_min;
+        res.hit_node = int(s.parent);
+        res.node_depth = kSMax - s.scale - 1;
+        return true;
+      }
+      ofs = pool.far_ptrs[s.child_descriptor >> 11];
+      if (COUNT) ++fetches;
+    } else {
+      ofs = (s.child_descriptor >> 11) + s.parent - (1u << 20);
+    }
+    ofs += __popc(s.child_descriptor & (child_masks - 1u) & 255u);
+    if (ofs >= pool.nodes || s.scale < 1) {
+      res.fault_block = -1;
+      res.hit_duplicate = 0;
+      res.error_code = 2;
+      res.hit_t = s.t_min;
+      res.hit_node = int(s.parent);
+      res.node_depth = kSMax - s.scale - 1;
+      return true;
+    }
+    s.child_descriptor = pool.links[ofs];
+    if (COUNT) ++fetches;
+
+    if (!(s.child_descriptor & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2) {
+      res.fault_block = -1;
+      res.error_code = 0;
+      res.hit_t = s.t_min;
+      res.hit_node = int(ofs);
+      res.hit_duplicate = int((s.child_descriptor >> 9) & 1u);
+      res.node_depth = kSMax - s.scale;
+      return true;
+    }
+
+    const float half = s.scale_exp2 * 0.5f;
+    const float tx_center = half * s.tx_coef + tx_corner;
+    const float ty_center = half * s.ty_coef + ty_corner;
+    const float tz_center = half * s.tz_coef + tz_corner;
+
+    if (tc_max < s.h) stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)));
+    s.h = tc_max;
+    s.parent = ofs;
+
+    s.idx = 0;
+    s.scale--;
+    s.scale_exp2 = half;
+    if (tx_center > s.t_min) s.idx ^= 1u, s.posx += s.scale_exp2;
+    if (ty_center > s.t_min) s.idx ^= 2u, s.posy += s.scale_exp2;
+    if (tz_center > s.t_min) s.idx ^= 4u, s.posz += s.scale_exp2;
+    s.t_max = tc_max;
+    return false;
+  }
+
+  uint32_t step_mask = 0;
+  if (tx_corner <= tc_max) step_mask ^= 1u, s.posx -= s.scale_exp2;
+  if (ty_corner <= tc_max) step_mask ^= 2u, s.posy -= s.scale_exp2;
+  if (tz_corner <= tc_max) step_mask ^= 4u, s.posz -= s.scale_exp2;
+  s.t_min = tc_max;
+  s.idx ^= step_mask;
+
+  if ((s.idx & step_mask) != 0) {
+    uint32_t differing_bits = 0;
+    if (step_mask & 1u) differing_bits |= __float_as_uint(s.posx) ^ __float_as_uint(s.posx + s.scale_exp2);
+    if (step_mask & 2u) differing_bits |= __float_as_uint(s.posy) ^ __float_as_uint(s.posy + s.scale_exp2);
+    if (step_mask & 4u) differing_bits |= __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2);
+    // Highest set bit. The reference reads it from the exponent of
+    // (float)differing_bits; the two agree whenever the result is below
+    // S_MAX (the value is then exact in fp32), and both leave the loop
+    // otherwise.
+    s.scale = 31 - __clz(differing_bits);
+    if (s.scale >= kSMax) return false;  // left the root cube; the next step reports the miss
+    s.scale_exp2 = __uint_as_float(uint32_t(s.scale - kSMax + 127) << 23);
+
+    const uint2 e = stack.get(s.scale);
+    s.parent = e.x;
+    s.t_max = __uint_as_float(e.y);
+
+    const uint32_t shx = __float_as_uint(s.posx) >> s.scale;
+    const uint32_t shy = __float_as_uint(s.posy) >> s.scale;
+    const uint32_t shz = __float_as_uint(s.posz) >> s.scale;
+    s.posx = __uint_as_float(shx << s.scale);
+    s.posy = __uint_as_float(shy << s.scale);
+    s.posz = __uint_as_float(shz << s.scale);
+    s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
+    s.h = 0.0f;
+    s.child_descriptor = 0;
+  }
+  return false;
+}
+
+template <class Stack, bool COUNT>
+__device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node, float px, float py, float pz,
+                                        float dx, float dy, float dz, float ray_size_coef, float ray_size_bias,
+                                        Stack& stack, CastResult& res, uint32_t& fetches) {
+  Trav s;
+  TravInit(s, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
+  while (!TravStep<Stack, COUNT>(s, pool, stack, res, fetches)) {
+  }
 }
 
 // mul(float16, float3) of init_tracing_frame.cl:3-9: (x,y,z,1) against rows,
