@@ -57,7 +57,7 @@ def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: 
     flags = api.L.RT_FLAG_SHADOWS
     specs = view_mix(nviews, seed)
     b = api.BatchRenderer(cache, nviews)
-    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=4, tree_depth=depth)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=depth)
     views, total, offsets = b.make_views(specs, flags=flags)
     out = api.Buffer(ctx, 4 * total)
     codes = api.Buffer(ctx, 4 * total)
@@ -66,7 +66,7 @@ def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: 
     t = time.time()
     best, best_ms = b.autotune(views, nviews, out, codes, repeats=2)
     log(f"batched: {nviews} views, {total} pixels, {rays} rays; autotune {time.time() - t:.1f}s -> "
-        f"tile {api.BatchRenderer.TILE_SHAPES[best[0]]}, {best[1]} LDS levels, {best[2]} WG/CU ({best_ms:.2f} ms)")
+        f"tile {api.BatchRenderer.TILE_SHAPES[best[0]]}, {best[1]} LDS levels, {best[2]} waves/CU ({best_ms:.2f} ms)")
     ms = []
     for _ in range(steps):
         b.render(views, nviews, out, codes)
@@ -161,6 +161,23 @@ def main():
     bytes_per_launch = 4 * fetches + 8 * hits + 8 * primary  # links+far pointers, normal+colour, pixel+code
     log(f"frame: {primary} primary + {shadow} shadow rays, {fetches} node fetches ({fetches / rays:.1f}/ray), {hits} hits")
 
+    # Launch shape of the plain kernel: the fastest of a small sweep (outside the timed region).
+    pout, pcodes = api.Buffer(ctx, 4 * W * H), api.Buffer(ctx, 4 * W * H)
+    shapes = []
+    for waves in (1, 4):
+        for levels in (10, 12, 16):
+            api.plain_config(ctx, levels, waves, args.depth)
+            api.render_plain(ctx, cache, W, H, vpi, lod=lod, flags=flags, out=pout, codes=pcodes)
+            tm = api.EventTimer(ctx)
+            tm.start()
+            for _ in range(5):
+                api.render_plain(ctx, cache, W, H, vpi, lod=lod, flags=flags, out=pout, codes=pcodes)
+            shapes.append((tm.stop() / 5, waves, levels))
+    shapes.sort()
+    plain_ms, plain_waves, plain_levels = shapes[0]
+    api.plain_config(ctx, plain_levels, plain_waves, args.depth)
+    log("plain launch shapes (ms, waves/WG, LDS levels): " + ", ".join(f"({m:.3f}, {w}, {l})" for m, w, l in shapes))
+
     tr = api.GPURayTracer(cache, W, H)
     tr.set_frame_time_limit(1e9)
     tr.set_lod_voxel_size(lod)
@@ -227,7 +244,8 @@ def main():
         "config": {"workload": "plain: depth-11 fBm terrain + 64 boxes, resident, 1920x1080, primary + shadow rays",
                    "width": W, "height": H, "depth": args.depth, "nodes_in_block": 8192, "lod_voxel_size": lod,
                    "blocks": info.blocks_count, "nodes": info.source_nodes, "rays_per_frame": rays,
-                   "primary_rays": primary, "shadow_rays": shadow},
+                   "primary_rays": primary, "shadow_rays": shadow,
+                   "plain_launch": {"waves_per_workgroup": plain_waves, "lds_stack_levels": plain_levels}},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "RenderPlainKernel", "kernel_ms": round(avg_kernel_ms, 4),

@@ -129,9 +129,11 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                     const rt_buf* normals, const rt_buf* colors, int nodes_in_block,
                     const float view_proj_inv[16], float lod_voxel_size, const float background[4], int flags,
                     uint64_t* stats);
-/* Launch-shape control for the plain kernel: LDS stack levels (8..22) and waves per SIMD hint.
- * Applies to subsequent rt_render_plain calls on this context. */
-int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int tile_shape);
+/* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context:
+ * lds_stack_levels (8, 10, 12, 14, 16, 22) traversal-stack levels held in LDS, wg_waves 1 (one wave per
+ * workgroup, 8x8 pixels) or 4 (16x16 pixels), tree_depth of the scene (0 = unknown) to keep the deepest
+ * levels in LDS. Default: 16 levels, 4 waves. Results do not depend on the shape. */
+int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int wg_waves, int tree_depth);
 
 /* ------------------------------------------------------ GPU algorithms
  * NodeToBlockKernel: GPU_renderer/kernels/feedback_compression.cl:4-13. */
@@ -272,14 +274,18 @@ typedef struct rt_view {
   float view_proj_inv[16];
   int width, height;
   float lod_voxel_size;
-  int flags; /* RT_FLAG_SHADOWS */
-  uint64_t pixel_offset;
+  int flags;             /* RT_FLAG_SHADOWS */
+  int scene;             /* 0: the cache given to rt_batch_create; else an index from rt_batch_add_scene */
+  uint64_t pixel_offset; /* first pixel of this view in the output buffers */
 } rt_view;
 
+/* The batch renders scene 0 = `cache`; every scene must stay resident (no faults) while batches run. */
 int rt_batch_create(rt_cache* cache, int max_views, rt_batch** out);
 int rt_batch_destroy(rt_batch* batch);
+/* Registers another resident scene of the same context; returns its scene index (>= 1) or a negative error. */
+int rt_batch_add_scene(rt_batch* batch, rt_cache* cache);
 /* Launch shape: tile_shape 0 = 8x8, 1 = 16x4, 2 = 32x2 pixels per wave; lds_levels (8, 10, 12, 16) deepest
- * traversal-stack levels kept in LDS; wgs_per_cu workgroups of 256 lanes per compute unit; tree_depth of the
+ * traversal-stack levels kept in LDS; wgs_per_cu one-wave workgroups per compute unit (1..32); tree_depth of the
  * scene (places the LDS window on the deepest levels). */
 int rt_batch_config(rt_batch* batch, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth);
 /* Enqueues one batched launch on the cache's context stream. stats (may be NULL) makes it the counting variant
@@ -293,6 +299,79 @@ int rt_batch_last_ms(rt_batch* batch, double* ms);
  * best = {tile_shape, lds_levels, wgs_per_cu} with its time. */
 int rt_batch_autotune(rt_batch* batch, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes,
                       int repeats, int best[3], double* best_ms);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous-batching request loop (no reference counterpart: the reference
+ * renders one view per RenderFrame on the calling thread). A server thread
+ * takes submitted requests in FIFO order and admits each when
+ *  - its output fits in the device framebuffer arena (8 bytes per pixel:
+ *    RGBA8 + feedback code), and
+ *  - its scene is resident: scenes are loaded whole into their own block
+ *    pool on first use, within a device pool budget, evicting idle scenes
+ *    (no request in flight) least recently used first.
+ * Admitted requests form a round: one persistent batched launch
+ * (rt_batch_render semantics; each result equals rt_render_plain of that
+ * view) on one of `rounds_in_flight` streams, followed by the copy of each
+ * result to the host. Requests that arrive while rounds run are admitted into
+ * the next round as soon as a round slot, arena space and pool space free up;
+ * rounds on different streams overlap on the device. */
+typedef struct rt_server_config {
+  int max_views_per_round;    /* 1..65536 */
+  int rounds_in_flight;       /* 1..4 */
+  uint64_t framebuffer_bytes; /* device arena for request outputs */
+  uint64_t pool_bytes;        /* device budget for resident scenes' block pools */
+  int tile_shape, lds_levels, wgs_per_cu; /* launch shape of every round (rt_batch_config) */
+} rt_server_config;
+
+typedef struct rt_request {
+  float view_proj_inv[16];
+  int width, height;
+  float lod_voxel_size;
+  int flags;           /* RT_FLAG_SHADOWS */
+  int scene;           /* index returned by rt_server_add_scene */
+  uint32_t* out_image; /* host, width*height RGBA8 words, or NULL; must stay valid until completion */
+  int32_t* out_codes;  /* host, width*height feedback codes, or NULL */
+} rt_request;
+
+typedef struct rt_completion {
+  uint64_t id;
+  int status;          /* RT_OK, or the error that failed the request's round */
+  int round_views;     /* views in the round that rendered it */
+  double submit_ms;    /* times on the server's clock (ms since rt_server_create) */
+  double admit_ms;
+  double done_ms;      /* result in host memory */
+} rt_completion;
+
+typedef struct rt_server_stats {
+  uint64_t submitted, completed, failed, rounds;
+  uint64_t pixels, primary_rays, shadow_rays;
+  uint64_t scene_loads, scene_evictions;
+  int max_rounds_in_flight; /* largest number of rounds seen on the device together */
+  int pad;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* request latency, submit -> result in host memory */
+  double span_ms;       /* first admission -> last completion since the last reset */
+  double mrays_per_s;   /* (primary + shadow rays) / span */
+  double mean_views_per_round;
+} rt_server_stats;
+
+/* Starts the server thread. */
+int rt_server_create(rt_ctx* ctx, const rt_server_config* cfg, rt_server** out);
+/* Stops the thread after the requests in flight are done; pending requests complete as failed. */
+int rt_server_destroy(rt_server* server);
+/* Registers a scene (the tree must outlive the server); tree_depth places the LDS stack window (0 = 11).
+ * Returns the scene index or a negative error (RT_ERR_RANGE when its block pool exceeds pool_bytes). */
+int rt_server_add_scene(rt_server* server, rt_tree* tree, int tree_depth);
+/* Queues a request (thread-safe, does not wait); rejects requests that can never be admitted. */
+int rt_server_submit(rt_server* server, const rt_request* req, uint64_t* id);
+/* Queues n requests at once (all or none are accepted); ids receives their ids in order. */
+int rt_server_submit_many(rt_server* server, const rt_request* reqs, int n, uint64_t* ids);
+/* Returns up to cap completions, waiting up to timeout_ms for the first one; the count or a negative error. */
+int rt_server_poll(rt_server* server, rt_completion* out, int cap, int timeout_ms);
+/* Waits until every submitted request has completed (or timeout_ms passes: RT_ERR_STATE). */
+int rt_server_drain(rt_server* server, int timeout_ms);
+int rt_server_stats_get(rt_server* server, rt_server_stats* out);
+/* Clears latencies and counters (for measuring after a warm-up). */
+int rt_server_reset_stats(rt_server* server);
 
 /* ----------------------------------------------------------------- camera */
 /* Camera::ViewProjectionMatrix().Inverse(): scene/camera.cpp:23-28, util/mat.cpp:6-194. */

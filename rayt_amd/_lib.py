@@ -69,7 +69,7 @@ SIGNATURES = {
     "rt_raytrace_restart": (I, [P, P, I, I, P, I, FP, F]),
     "rt_legacy_links_from_cache": (I, [P, P, P, P, U64]),
     "rt_render_plain": (I, [P, P, P, I, I, P, P, I, P, P, I, FP, F, FP, I, U64P]),
-    "rt_plain_config": (I, [P, I, I]),
+    "rt_plain_config": (I, [P, I, I, I]),
     "rt_node_to_block": (I, [P, I, I, P]),
     "rt_difference_flag": (I, [P, I, P, P]),
     "rt_compress": (I, [P, I, P, P, P]),
@@ -134,12 +134,44 @@ SIGNATURES = {
     "rt_batch_render": (I, [P, C.c_void_p, I, P, P, U64P]),
     "rt_batch_last_ms": (I, [P, C.POINTER(D)]),
     "rt_batch_autotune": (I, [P, C.c_void_p, I, P, P, I, IP, C.POINTER(D)]),
+    "rt_batch_add_scene": (I, [P, P]),
+    "rt_server_create": (I, [P, C.c_void_p, PP]),
+    "rt_server_destroy": (I, [P]),
+    "rt_server_add_scene": (I, [P, P, I]),
+    "rt_server_submit": (I, [P, C.c_void_p, U64P]),
+    "rt_server_submit_many": (I, [P, C.c_void_p, I, U64P]),
+    "rt_server_poll": (I, [P, C.c_void_p, I, I]),
+    "rt_server_drain": (I, [P, I]),
+    "rt_server_stats_get": (I, [P, C.c_void_p]),
+    "rt_server_reset_stats": (I, [P]),
 }
 
 
 class View(C.Structure):
     _fields_ = [("view_proj_inv", F * 16), ("width", I), ("height", I), ("lod_voxel_size", F), ("flags", I),
-                ("pixel_offset", U64)]
+                ("scene", I), ("pixel_offset", U64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_views_per_round", I), ("rounds_in_flight", I), ("framebuffer_bytes", U64),
+                ("pool_bytes", U64), ("tile_shape", I), ("lds_levels", I), ("wgs_per_cu", I)]
+
+
+class Request(C.Structure):
+    _fields_ = [("view_proj_inv", F * 16), ("width", I), ("height", I), ("lod_voxel_size", F), ("flags", I),
+                ("scene", I), ("out_image", C.c_void_p), ("out_codes", C.c_void_p)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", U64), ("status", I), ("round_views", I), ("submit_ms", D), ("admit_ms", D),
+                ("done_ms", D)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("rounds", U64), ("pixels", U64),
+                ("primary_rays", U64), ("shadow_rays", U64), ("scene_loads", U64), ("scene_evictions", U64),
+                ("max_rounds_in_flight", I), ("pad", I), ("p50_ms", D), ("p99_ms", D), ("mean_ms", D),
+                ("max_ms", D), ("span_ms", D), ("mrays_per_s", D), ("mean_views_per_round", D)]
 
 _lib = None
 

@@ -451,6 +451,34 @@ class Camera:
         return out
 
 
+def plain_config(ctx: Context, lds_levels: int = 16, wg_waves: int = 4, tree_depth: int = 0):
+    """Launch shape of the plain kernel on this context (rt_plain_config)."""
+    call("rt_plain_config", ctx.h, lds_levels, wg_waves, tree_depth)
+
+
+class EventTimer:
+    """Device time between two events recorded on the context stream."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.a = self.b = None
+
+    def start(self):
+        self.a = C.c_void_p()
+        call("rt_evt_record", self.ctx.h, C.byref(self.a))
+
+    def stop(self) -> float:
+        self.b = C.c_void_p()
+        call("rt_evt_record", self.ctx.h, C.byref(self.b))
+        arr = (C.c_void_p * 1)(self.b)
+        call("rt_evt_wait", 1, arr)
+        ns = C.c_double()
+        call("rt_evt_elapsed_ns", self.a, self.b, C.byref(ns))
+        lib().rt_evt_release(self.a)
+        lib().rt_evt_release(self.b)
+        return ns.value * 1e-6
+
+
 def render_plain(ctx: Context, cache: GPUOctreeCacheManager, width: int, height: int, vpi: np.ndarray,
                  lod: float = 0.0, flags: int = 0, out: Buffer | None = None, codes: Buffer | None = None,
                  count: bool = False):
@@ -478,19 +506,27 @@ class BatchRenderer:
         call("rt_batch_create", cache.h, max_views, C.byref(h))
         self.h, self.cache, self.ctx = h, cache, cache.ctx
 
-    def config(self, tile_shape: int = 0, lds_levels: int = 12, wgs_per_cu: int = 4, tree_depth: int = 11):
+    def config(self, tile_shape: int = 0, lds_levels: int = 12, wgs_per_cu: int = 16, tree_depth: int = 11):
         call("rt_batch_config", self.h, tile_shape, lds_levels, wgs_per_cu, tree_depth)
+
+    def add_scene(self, cache: GPUOctreeCacheManager) -> int:
+        """Registers another resident scene; returns its index for make_views."""
+        idx = call("rt_batch_add_scene", self.h, cache.h)
+        self._scenes = getattr(self, "_scenes", []) + [cache]  # keep the caches alive
+        return idx
 
     @staticmethod
     def make_views(specs, flags: int = 0, lod: float = 0.0):
-        """specs: list of (view_proj_inv, width, height). Returns (ctypes array, total pixels, offsets)."""
+        """specs: list of (view_proj_inv, width, height[, scene]). Returns (ctypes array, total pixels, offsets)."""
         arr = (L.View * len(specs))()
         off, offsets = 0, []
-        for i, (vpi, w, h) in enumerate(specs):
+        for i, spec in enumerate(specs):
+            vpi, w, h = spec[:3]
             arr[i].view_proj_inv[:] = [float(x) for x in np.asarray(vpi, np.float32).reshape(16)]
             arr[i].width, arr[i].height = w, h
             arr[i].lod_voxel_size = lod
             arr[i].flags = flags
+            arr[i].scene = spec[3] if len(spec) > 3 else 0
             arr[i].pixel_offset = off
             offsets.append(off)
             off += w * h
@@ -518,6 +554,85 @@ class BatchRenderer:
     def close(self):
         if self.h:
             lib().rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RenderServer:
+    """Continuous-batching request loop (rt_server_*): FIFO admission by framebuffer-arena and block-pool
+    memory, rounds of batched launches overlapping on several streams, p50/p99 latency."""
+
+    def __init__(self, ctx: Context, max_views_per_round: int = 64, rounds_in_flight: int = 2,
+                 framebuffer_bytes: int = 1 << 30, pool_bytes: int = 16 << 30, tile_shape: int = 0,
+                 lds_levels: int = 12, wgs_per_cu: int = 16):
+        cfg = L.ServerConfig(max_views_per_round, rounds_in_flight, framebuffer_bytes, pool_bytes, tile_shape,
+                             lds_levels, wgs_per_cu)
+        h = C.c_void_p()
+        call("rt_server_create", ctx.h, C.byref(cfg), C.byref(h))
+        self.h, self.ctx = h, ctx
+        self._trees = []
+        self._live = {}  # request id -> (request struct, image array, codes array)
+
+    def add_scene(self, tree: StoredOctree, depth: int = 0) -> int:
+        idx = call("rt_server_add_scene", self.h, tree.h, depth)
+        self._trees.append(tree)
+        return idx
+
+    def submit(self, vpi, width: int, height: int, scene: int = 0, flags: int = 0, lod: float = 0.0,
+               want_image: bool = True, want_codes: bool = False) -> int:
+        return self.submit_many([(vpi, width, height, scene)], flags, lod, want_image, want_codes)[0]
+
+    def submit_many(self, specs, flags: int = 0, lod: float = 0.0, want_image: bool = True,
+                    want_codes: bool = False) -> list[int]:
+        """specs: list of (view_proj_inv, width, height, scene), queued together. Returns request ids."""
+        n = len(specs)
+        arr = (L.Request * n)()
+        bufs = []
+        for i, (vpi, width, height, scene) in enumerate(specs):
+            img = np.empty(width * height, np.uint32) if want_image else None
+            cod = np.empty(width * height, np.int32) if want_codes else None
+            r = arr[i]
+            r.view_proj_inv[:] = [float(x) for x in np.asarray(vpi, np.float32).reshape(16)]
+            r.width, r.height, r.lod_voxel_size, r.flags, r.scene = width, height, lod, flags, scene
+            r.out_image = img.ctypes.data if img is not None else None
+            r.out_codes = cod.ctypes.data if cod is not None else None
+            bufs.append((img, cod))
+        ids = (C.c_uint64 * n)()
+        call("rt_server_submit_many", self.h, C.cast(arr, C.c_void_p), n, ids)
+        for i in range(n):
+            self._live[ids[i]] = (None, *bufs[i])
+        return list(ids)
+
+    def poll(self, cap: int = 256, timeout_ms: int = 0):
+        """Returns a list of (completion struct, image or None, codes or None)."""
+        arr = (L.Completion * cap)()
+        n = call("rt_server_poll", self.h, C.cast(arr, C.c_void_p), cap, timeout_ms)
+        out = []
+        for i in range(n):
+            c = arr[i]
+            _, img, cod = self._live.pop(c.id, (None, None, None))
+            out.append((c, img, cod))
+        return out
+
+    def drain(self, timeout_ms: int = 60000):
+        call("rt_server_drain", self.h, timeout_ms)
+
+    def stats(self) -> L.ServerStats:
+        s = L.ServerStats()
+        call("rt_server_stats_get", self.h, C.byref(s))
+        return s
+
+    def reset_stats(self):
+        call("rt_server_reset_stats", self.h)
+
+    def close(self):
+        if self.h:
+            lib().rt_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

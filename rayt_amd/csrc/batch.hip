@@ -1,29 +1,46 @@
-// Batched rendering: many views of any resolution traced by one persistent
-// kernel (no reference counterpart; per view the result is the plain
-// single-pass frame of trace.hip, bit for bit).
+// Batched rendering: many views of any resolution, over one or several
+// resident scenes, traced by one persistent kernel (no reference
+// counterpart; per view the result is the plain single-pass frame of
+// trace.hip, bit for bit).
 //
 // Work layout. Every view is cut into wave tiles of 64 pixels (8x8, 16x4 or
 // 32x2); the tiles of all views form one global sequence, split into eight
-// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8). A wave
-// claims a group of tiles from its XCD's range with one atomic, renders them,
-// and when its range is exhausted steals from the next XCD's range; after all
-// eight ranges are exhausted every wave exits, so the grid always drains.
-// The traversal stack keeps the deepest `lds_levels` levels in LDS.
+// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8). Workgroups
+// are single waves. A wave claims groups of tiles from its XCD's range with
+// one atomic and, when that range is exhausted, steals from the next XCD's;
+// after all eight ranges are exhausted every wave exits, so the grid drains.
+// Inside a wave, a lane whose ray (primary, then shadow) has finished pulls
+// the next pixel of the wave's tiles, so lanes stay busy while the slowest
+// ray of a tile is still being traced. The traversal stack keeps the deepest
+// `lds_levels` levels in LDS.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
+#include "batch.h"
 #include "cache.h"
 #include "render_pixel.h"
-#include "rt_internal.h"
 #include "trace_common.h"
 
 using namespace rtdev;
 
 namespace {
 
-constexpr int kMaxBatchViews = 1024;
 constexpr uint32_t kTilesPerGrab = 4;
+// A refill happens when at least this many lanes are idle, so its
+// bookkeeping is paid once per several traversal steps.
+constexpr int kRefillMin = 16;
+
+struct DevScene {
+  const uint32_t* links;
+  const uint32_t* far_ptrs;
+  const uint32_t* normals;
+  const uint32_t* colors;
+  uint32_t nodes, far_count, root;
+  int nodes_in_block;
+  int lds_base;  // first traversal-stack level kept in LDS for this scene's depth
+  int pad;
+};
 
 struct DevView {
   float m[16];
@@ -31,7 +48,7 @@ struct DevView {
   float lod;
   int flags;
   int tiles_x;
-  int pad;
+  int scene;
   uint64_t pixel_offset;
 };
 
@@ -39,28 +56,17 @@ struct Vec4B {
   float v[4];
 };
 
-// Lanes that need a ray pull the next pixel of the wave's chunk; a chunk is
-// kTilesPerGrab wave tiles claimed from the XCD queues. Refills happen when
-// at least kRefillMin lanes are idle, so the cost of the bookkeeping is paid
-// once per several traversal steps.
-constexpr int kRefillMin = 16;
-
 template <int TW, int LDS_LEVELS, bool COUNT>
-__global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ views, int nviews,
-                                                   const uint32_t* __restrict__ tile_begin, uint32_t total_tiles,
-                                                   uint32_t* __restrict__ counters, uint32_t* __restrict__ out,
-                                                   int* __restrict__ codes, PoolView pool, uint32_t root,
-                                                   const uint32_t* __restrict__ normals,
-                                                   const uint32_t* __restrict__ colors, int nodes_in_block, Vec4B bg,
-                                                   Lights lt, int lds_base, unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ views, int nviews,
+                                                  const DevScene* __restrict__ scenes,
+                                                  const uint32_t* __restrict__ tile_begin, uint32_t total_tiles,
+                                                  uint32_t* __restrict__ counters, uint32_t* __restrict__ out,
+                                                  int* __restrict__ codes, Vec4B bg, Lights lt,
+                                                  unsigned long long* __restrict__ stats) {
   constexpr int TH = 64 / TW;
-  __shared__ uint2 lds[LDS_LEVELS * 256];
-  __shared__ uint32_t begin_s[kMaxBatchViews + 1];
-  for (int i = threadIdx.x; i <= nviews; i += blockDim.x) begin_s[i] = tile_begin[i];
-  __syncthreads();
-  SplitStack<LDS_LEVELS, 256> stack;
+  __shared__ uint2 lds[LDS_LEVELS * 64];
+  SplitStack<LDS_LEVELS, 64> stack;
   stack.lds = lds + threadIdx.x;
-  stack.base = lds_base;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   const uint32_t xcd = blockIdx.x & 7u;
@@ -76,6 +82,14 @@ __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ v
   bool active = false;
   int phase = 0;  // 0: primary ray, 1: shadow ray
   Trav tr;
+  PoolView pool;
+  pool.links = nullptr;
+  pool.far_ptrs = nullptr;
+  pool.nodes = pool.far_count = 0;
+  const uint32_t* normals = nullptr;
+  const uint32_t* colors = nullptr;
+  uint32_t root = 0;
+  int nib = 0;
   uint64_t pix = 0;
   int flags = 0, code = 0, hit_node = 0;
   float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
@@ -111,16 +125,26 @@ __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ v
         if (!active && rank < take) {
           const uint32_t p = chunk_pix + rank;
           const uint32_t tile = chunk_first + p / 64u, local_px = p % 64u;
-          int lo = 0, hi = nviews - 1;
+          int lo = 0, hi = nviews - 1;  // view holding the tile: last v with tile_begin[v] <= tile
           while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (begin_s[mid] <= tile) lo = mid; else hi = mid - 1;
+            if (tile_begin[mid] <= tile) lo = mid; else hi = mid - 1;
           }
           const DevView& v = views[lo];
-          const uint32_t local = tile - begin_s[lo];
+          const uint32_t local = tile - tile_begin[lo];
           const int x = int(local % uint32_t(v.tiles_x)) * TW + int(local_px % TW);
           const int y = int(local / uint32_t(v.tiles_x)) * TH + int(local_px / TW);
           if (x < v.width && y < v.height) {
+            const DevScene& sc = scenes[v.scene];
+            pool.links = sc.links;
+            pool.far_ptrs = sc.far_ptrs;
+            pool.nodes = sc.nodes;
+            pool.far_count = sc.far_count;
+            normals = sc.normals;
+            colors = sc.colors;
+            root = sc.root;
+            nib = sc.nodes_in_block;
+            stack.base = sc.lds_base;
             const PrimaryRay r = InitRay(v.m, x, y, v.width, v.height, v.lod);
             ox = r.origin.x, oy = r.origin.y, oz = r.origin.z;
             dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
@@ -142,7 +166,7 @@ __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ v
     }
     if (!active) continue;
     CastResult res;
-    if (!TravStep<SplitStack<LDS_LEVELS, 256>, COUNT>(tr, pool, stack, res, fetches)) continue;
+    if (!TravStep<SplitStack<LDS_LEVELS, 64>, COUNT>(tr, pool, stack, res, fetches)) continue;
     uint32_t color;
     if (phase == 0) {
       if (res.error_code) {
@@ -155,7 +179,7 @@ __global__ __launch_bounds__(256) void BatchKernel(const DevView* __restrict__ v
         color = 0u;
         code = kNoHit;
       } else {
-        code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);
+        code = HitCode(res.hit_node, res.hit_duplicate, nib);
         hit_node = res.hit_node;
         if (COUNT) ++cnt.hits;
         bool start_shadow = false;
@@ -195,24 +219,34 @@ __global__ void ZeroCountersKernel(uint32_t* c, int n) {
   if (int(threadIdx.x) < n) c[threadIdx.x] = 0u;
 }
 
+struct LaunchArgs {
+  dim3 grid;
+  hipStream_t stream;
+  const DevView* views;
+  int nviews;
+  const DevScene* scenes;
+  const uint32_t* tile_begin;
+  uint32_t total_tiles;
+  uint32_t* counters;
+  uint32_t* out;
+  int* codes;
+  Vec4B bg;
+  Lights lt;
+  unsigned long long* stats;
+};
+
 template <int TW, int L, bool C>
-void Launch(dim3 grid, hipStream_t s, const DevView* views, int nviews, const uint32_t* tb, uint32_t total,
-            uint32_t* counters, uint32_t* out, int* codes, const PoolView& pool, uint32_t root, const uint32_t* nrm,
-            const uint32_t* col, int nib, const Vec4B& bg, const Lights& lt, int base, unsigned long long* st) {
-  BatchKernel<TW, L, C><<<grid, dim3(256), 0, s>>>(views, nviews, tb, total, counters, out, codes, pool, root, nrm,
-                                                   col, nib, bg, lt, base, st);
+void Launch(const LaunchArgs& a) {
+  BatchKernel<TW, L, C><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles,
+                                                           a.counters, a.out, a.codes, a.bg, a.lt, a.stats);
 }
 
 template <bool C>
-bool Dispatch(int tw, int levels, dim3 grid, hipStream_t s, const DevView* views, int nviews, const uint32_t* tb,
-              uint32_t total, uint32_t* counters, uint32_t* out, int* codes, const PoolView& pool, uint32_t root,
-              const uint32_t* nrm, const uint32_t* col, int nib, const Vec4B& bg, const Lights& lt, int base,
-              unsigned long long* st) {
-#define RT_B(TW, L)                                                                                            \
-  if (tw == TW && levels == L) {                                                                               \
-    Launch<TW, L, C>(grid, s, views, nviews, tb, total, counters, out, codes, pool, root, nrm, col, nib, bg, lt, \
-                     base, st);                                                                                \
-    return true;                                                                                               \
+bool Dispatch(int tw, int levels, const LaunchArgs& a) {
+#define RT_B(TW, L)                   \
+  if (tw == TW && levels == L) {      \
+    Launch<TW, L, C>(a);              \
+    return true;                      \
   }
   RT_B(8, 8) RT_B(8, 10) RT_B(8, 12) RT_B(8, 16)
   RT_B(16, 8) RT_B(16, 10) RT_B(16, 12) RT_B(16, 16)
@@ -223,90 +257,107 @@ bool Dispatch(int tw, int levels, dim3 grid, hipStream_t s, const DevView* views
 
 int TileWidth(int shape) { return shape == 1 ? 16 : shape == 2 ? 32 : 8; }
 
+DevScene SceneOf(rt_cache* c, int tree_depth, int lds_levels) {
+  rayt::BlockCacheManager& m = *c->mgr;
+  rt_buf* links = rt_cache_channel(c, 0);
+  rt_buf* far_ptrs = rt_cache_far_pointers(c);
+  rt_buf* normals = rt_cache_channel_by_name(c, "normals");
+  rt_buf* colors = rt_cache_channel_by_name(c, "colors");
+  DevScene s;
+  s.links = static_cast<const uint32_t*>(links->ptr);
+  s.far_ptrs = static_cast<const uint32_t*>(far_ptrs->ptr);
+  s.normals = normals ? static_cast<const uint32_t*>(normals->ptr) : nullptr;
+  s.colors = colors ? static_cast<const uint32_t*>(colors->ptr) : nullptr;
+  s.nodes = uint32_t(std::min<uint64_t>(links->size / 4, 0xffffffffu));
+  s.far_count = uint32_t(std::min<uint64_t>(far_ptrs->size / 4, 0xffffffffu));
+  s.root = uint32_t(m.root_node_index());
+  s.nodes_in_block = m.nodes_in_block();
+  // Deepest stack levels in LDS: a depth-D tree pushes levels 0 .. D-2.
+  s.lds_base = std::max(0, (tree_depth - 1) - lds_levels);
+  s.pad = 0;
+  return s;
+}
+
 }  // namespace
 
 struct rt_batch {
-  rt_cache* cache = nullptr;
   rt_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;  // the context stream unless the server assigns one
   int max_views = 0;
   int tile_shape = 0;   // 0: 8x8, 1: 16x4, 2: 32x2
   int lds_levels = 12;  // 8, 10, 12 or 16
-  int wgs_per_cu = 4;
-  int lds_base = -1;    // -1: derive from tree_depth
-  int tree_depth = 11;
+  int wgs_per_cu = 16;  // one-wave workgroups per compute unit
+  int tree_depth = 11;  // depth assumed for scenes registered without one
+  std::vector<rt_cache*> scenes;
+  std::vector<int> depths;  // per scene; 0 = tree_depth
+  DevScene* d_scenes = nullptr;
   DevView* d_views = nullptr;
   uint32_t* d_tile_begin = nullptr;
   uint32_t* d_counters = nullptr;
   unsigned long long* d_stats = nullptr;
-  std::vector<DevView> h_views;
-  std::vector<uint32_t> h_tile_begin;
+  // Pinned descriptor staging, rewritten only after the previous launch has completed.
+  DevView* h_views = nullptr;
+  uint32_t* h_tile_begin = nullptr;
+  DevScene* h_scenes = nullptr;
+  unsigned long long* h_stats = nullptr;
+  bool launched = false;
+  bool counted = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0;
 };
 
-extern "C" {
+namespace rayt {
 
-int rt_batch_create(rt_cache* cache, int max_views, rt_batch** out) {
-  RT_REQUIRE(cache && out, RT_ERR_INVALID, "null argument");
-  RT_REQUIRE(max_views > 0 && max_views <= kMaxBatchViews, RT_ERR_RANGE, "max_views must be in [1, 1024]");
+int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
+  RT_REQUIRE(ctx && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(max_views > 0 && max_views <= (1 << 16), RT_ERR_RANGE, "max_views must be in [1, 65536]");
   rt_batch* b = new rt_batch();
-  b->cache = cache;
-  b->ctx = cache->ctx;
+  b->ctx = ctx;
+  b->stream = ctx->stream;
   b->max_views = max_views;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(DevView) * max_views);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_tile_begin), 4 * (max_views + 1));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counters), 4 * 8);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_stats), 8 * 4);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_scenes), sizeof(DevScene) * kMaxBatchScenes);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_views), sizeof(DevView) * max_views, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_tile_begin), 4 * (max_views + 1), 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_scenes), sizeof(DevScene) * kMaxBatchScenes, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stats), 8 * 4, 0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e != hipSuccess) {
     rt_batch_destroy(b);
-    return rayt::Fail(RT_ERR_HIP, std::string("batch allocation: ") + hipGetErrorString(e));
+    return Fail(RT_ERR_HIP, std::string("batch allocation: ") + hipGetErrorString(e));
   }
   *out = b;
   return RT_OK;
 }
 
-int rt_batch_destroy(rt_batch* b) {
-  if (!b) return RT_OK;
-  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-  (void)hipFree(b->d_views);
-  (void)hipFree(b->d_tile_begin);
-  (void)hipFree(b->d_counters);
-  (void)hipFree(b->d_stats);
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
-  delete b;
+void BatchSetStream(rt_batch* b, hipStream_t s) { b->stream = s ? s : b->ctx->stream; }
+
+hipEvent_t BatchDoneEvent(rt_batch* b) { return b->ev1; }
+
+int BatchSetScenes(rt_batch* b, const std::vector<rt_cache*>& caches, const std::vector<int>& depths) {
+  RT_REQUIRE(caches.size() == depths.size() && caches.size() <= size_t(kMaxBatchScenes), RT_ERR_RANGE,
+             "bad scene list");
+  b->scenes = caches;
+  b->depths = depths;
   return RT_OK;
 }
 
-int rt_batch_config(rt_batch* b, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth) {
-  RT_REQUIRE(b, RT_ERR_INVALID, "null batch");
-  RT_REQUIRE(tile_shape >= 0 && tile_shape <= 2, RT_ERR_RANGE, "tile_shape must be 0 (8x8), 1 (16x4) or 2 (32x2)");
-  RT_REQUIRE(lds_levels == 8 || lds_levels == 10 || lds_levels == 12 || lds_levels == 16, RT_ERR_RANGE,
-             "lds_levels must be 8, 10, 12 or 16");
-  RT_REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= 8, RT_ERR_RANGE, "wgs_per_cu must be in [1, 8]");
-  RT_REQUIRE(tree_depth >= 1 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [1, 22]");
-  b->tile_shape = tile_shape;
-  b->lds_levels = lds_levels;
-  b->wgs_per_cu = wgs_per_cu;
-  b->tree_depth = tree_depth;
-  return RT_OK;
-}
-
-int rt_batch_render(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes,
-                    uint64_t* stats) {
+int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes, bool count) {
   RT_REQUIRE(b && views && out_image, RT_ERR_INVALID, "null argument");
   RT_REQUIRE(nviews > 0 && nviews <= b->max_views, RT_ERR_RANGE, "nviews out of range");
-  rayt::BlockCacheManager& m = *b->cache->mgr;
+  RT_REQUIRE(!b->scenes.empty(), RT_ERR_STATE, "batch has no scene");
   const int tw = TileWidth(b->tile_shape), th = 64 / tw;
-  b->h_views.resize(nviews);
-  b->h_tile_begin.resize(nviews + 1);
+  // The staging below is read by the previous launch's copies: wait for it.
+  if (b->launched) RT_HIP_CHECK(hipEventSynchronize(b->ev1));
   uint64_t tiles = 0;
   for (int i = 0; i < nviews; ++i) {
     const rt_view& v = views[i];
     RT_REQUIRE(v.width > 0 && v.height > 0 && int64_t(v.width) * v.height <= (int64_t(1) << 28), RT_ERR_RANGE,
                "bad view size");
+    RT_REQUIRE(v.scene >= 0 && v.scene < int(b->scenes.size()), RT_ERR_RANGE, "view scene out of range");
     const uint64_t px = uint64_t(v.width) * v.height;
     RT_REQUIRE((v.pixel_offset + px) * 4 <= out_image->size && (!codes || (v.pixel_offset + px) * 4 <= codes->size),
                RT_ERR_RANGE, "view outside the output buffers");
@@ -317,63 +368,128 @@ int rt_batch_render(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_i
     d.lod = v.lod_voxel_size;
     d.flags = v.flags & RT_FLAG_SHADOWS;
     d.tiles_x = (v.width + tw - 1) / tw;
-    d.pad = 0;
+    d.scene = v.scene;
     d.pixel_offset = v.pixel_offset;
     b->h_tile_begin[i] = uint32_t(tiles);
     tiles += uint64_t(d.tiles_x) * ((v.height + th - 1) / th);
     RT_REQUIRE(tiles < (1ull << 31), RT_ERR_RANGE, "too many tiles in one batch");
   }
   b->h_tile_begin[nviews] = uint32_t(tiles);
-  hipStream_t s = b->ctx->stream;
-  RT_HIP_CHECK(hipMemcpyAsync(b->d_views, b->h_views.data(), sizeof(DevView) * nviews, hipMemcpyHostToDevice, s));
-  RT_HIP_CHECK(hipMemcpyAsync(b->d_tile_begin, b->h_tile_begin.data(), 4 * (nviews + 1), hipMemcpyHostToDevice, s));
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    rt_cache* c = b->scenes[i];
+    RT_REQUIRE(c->ctx == b->ctx, RT_ERR_INVALID, "scene belongs to another context");
+    RT_REQUIRE(c->mgr->root_node_index() >= 0, RT_ERR_STATE, "scene has no root block in its pool");
+    b->h_scenes[i] = SceneOf(c, b->depths[i] > 0 ? b->depths[i] : b->tree_depth, b->lds_levels);
+  }
+  hipStream_t s = b->stream;
+  RT_HIP_CHECK(hipMemcpyAsync(b->d_views, b->h_views, sizeof(DevView) * nviews, hipMemcpyHostToDevice, s));
+  RT_HIP_CHECK(hipMemcpyAsync(b->d_tile_begin, b->h_tile_begin, 4 * (nviews + 1), hipMemcpyHostToDevice, s));
+  RT_HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice,
+                              s));
   ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8);
-  if (stats) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8);
-  rt_buf* links = rt_cache_channel(b->cache, 0);
-  rt_buf* far_ptrs = rt_cache_far_pointers(b->cache);
-  rt_buf* normals = rt_cache_channel_by_name(b->cache, "normals");
-  rt_buf* colors = rt_cache_channel_by_name(b->cache, "colors");
-  PoolView pool;
-  pool.links = static_cast<const uint32_t*>(links->ptr);
-  pool.far_ptrs = static_cast<const uint32_t*>(far_ptrs->ptr);
-  pool.nodes = uint32_t(std::min<uint64_t>(links->size / 4, 0xffffffffu));
-  pool.far_count = uint32_t(std::min<uint64_t>(far_ptrs->size / 4, 0xffffffffu));
-  Vec4B bg;
+  if (count) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8);
+  LaunchArgs a;
+  a.stream = s;
+  a.views = b->d_views;
+  a.nviews = nviews;
+  a.scenes = b->d_scenes;
+  a.tile_begin = b->d_tile_begin;
+  a.total_tiles = uint32_t(tiles);
+  a.counters = b->d_counters;
+  a.out = static_cast<uint32_t*>(out_image->ptr);
+  a.codes = codes ? static_cast<int*>(codes->ptr) : nullptr;
   const float kBg[4] = {0.0f, float(216 / 255.), 1.0f, 1.0f};
-  memcpy(bg.v, kBg, sizeof(bg.v));
-  const Lights lt = rayt::MakeLights();
-  // Deepest stack levels in LDS: a depth-D tree pushes levels 0 .. D-2.
-  const int base = std::max(0, (b->tree_depth - 1) - b->lds_levels);
-  const uint32_t wgs = uint32_t(std::max(1, b->ctx->compute_units) * b->wgs_per_cu + 7) & ~7u;
-  const dim3 grid(wgs);
+  memcpy(a.bg.v, kBg, sizeof(a.bg.v));
+  a.lt = MakeLights();
+  // Enough single-wave workgroups for the work, at most wgs_per_cu per CU, a multiple of 8 (one range per XCD).
+  const uint64_t want = std::min<uint64_t>(uint64_t(std::max(1, b->ctx->compute_units)) * b->wgs_per_cu,
+                                           (tiles + kTilesPerGrab - 1) / kTilesPerGrab);
+  a.grid = dim3(uint32_t((std::max<uint64_t>(want, 1) + 7) & ~uint64_t(7)));
+  a.stats = count ? b->d_stats : nullptr;
   RT_HIP_CHECK(hipEventRecord(b->ev0, s));
-  const bool ok =
-      stats ? Dispatch<true>(TileWidth(b->tile_shape), b->lds_levels, grid, s, b->d_views, nviews, b->d_tile_begin,
-                             uint32_t(tiles), b->d_counters, static_cast<uint32_t*>(out_image->ptr),
-                             codes ? static_cast<int*>(codes->ptr) : nullptr, pool, uint32_t(m.root_node_index()),
-                             normals ? static_cast<const uint32_t*>(normals->ptr) : nullptr,
-                             colors ? static_cast<const uint32_t*>(colors->ptr) : nullptr, m.nodes_in_block(), bg, lt,
-                             base, b->d_stats)
-            : Dispatch<false>(TileWidth(b->tile_shape), b->lds_levels, grid, s, b->d_views, nviews, b->d_tile_begin,
-                              uint32_t(tiles), b->d_counters, static_cast<uint32_t*>(out_image->ptr),
-                              codes ? static_cast<int*>(codes->ptr) : nullptr, pool, uint32_t(m.root_node_index()),
-                              normals ? static_cast<const uint32_t*>(normals->ptr) : nullptr,
-                              colors ? static_cast<const uint32_t*>(colors->ptr) : nullptr, m.nodes_in_block(), bg,
-                              lt, base, nullptr);
+  const bool ok = count ? Dispatch<true>(tw, b->lds_levels, a) : Dispatch<false>(tw, b->lds_levels, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");
   RT_HIP_CHECK(hipGetLastError());
+  if (count) RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_stats, 8 * 4, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipEventRecord(b->ev1, s));
-  if (stats) {
-    unsigned long long h[4];
-    RT_HIP_CHECK(hipMemcpyAsync(h, b->d_stats, sizeof(h), hipMemcpyDeviceToHost, s));
-    RT_HIP_CHECK(hipStreamSynchronize(s));
-    for (int k = 0; k < 4; ++k) stats[k] = h[k];
-  }
+  b->launched = true;
+  b->counted = count;
   return RT_OK;
+}
+
+int BatchStats(rt_batch* b, uint64_t out[4]) {
+  RT_REQUIRE(b->launched && b->counted, RT_ERR_STATE, "last launch did not count");
+  RT_HIP_CHECK(hipEventSynchronize(b->ev1));
+  for (int k = 0; k < 4; ++k) out[k] = b->h_stats[k];
+  return RT_OK;
+}
+
+}  // namespace rayt
+
+extern "C" {
+
+int rt_batch_create(rt_cache* cache, int max_views, rt_batch** out) {
+  RT_REQUIRE(cache && out, RT_ERR_INVALID, "null argument");
+  rt_batch* b = nullptr;
+  const int rc = rayt::BatchCreate(cache->ctx, max_views, &b);
+  if (rc) return rc;
+  b->scenes.push_back(cache);
+  b->depths.push_back(0);
+  *out = b;
+  return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch* b) {
+  if (!b) return RT_OK;
+  if (b->launched) (void)hipEventSynchronize(b->ev1);
+  (void)hipFree(b->d_views);
+  (void)hipFree(b->d_tile_begin);
+  (void)hipFree(b->d_counters);
+  (void)hipFree(b->d_stats);
+  (void)hipFree(b->d_scenes);
+  if (b->h_views) (void)hipHostFree(b->h_views);
+  if (b->h_tile_begin) (void)hipHostFree(b->h_tile_begin);
+  if (b->h_scenes) (void)hipHostFree(b->h_scenes);
+  if (b->h_stats) (void)hipHostFree(b->h_stats);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  delete b;
+  return RT_OK;
+}
+
+int rt_batch_add_scene(rt_batch* b, rt_cache* cache) {
+  RT_REQUIRE(b && cache, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(cache->ctx == b->ctx, RT_ERR_INVALID, "scene belongs to another context");
+  RT_REQUIRE(int(b->scenes.size()) < rayt::kMaxBatchScenes, RT_ERR_RANGE, "too many scenes");
+  b->scenes.push_back(cache);
+  b->depths.push_back(0);
+  return int(b->scenes.size()) - 1;
+}
+
+int rt_batch_config(rt_batch* b, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth) {
+  RT_REQUIRE(b, RT_ERR_INVALID, "null batch");
+  RT_REQUIRE(tile_shape >= 0 && tile_shape <= 2, RT_ERR_RANGE, "tile_shape must be 0 (8x8), 1 (16x4) or 2 (32x2)");
+  RT_REQUIRE(lds_levels == 8 || lds_levels == 10 || lds_levels == 12 || lds_levels == 16, RT_ERR_RANGE,
+             "lds_levels must be 8, 10, 12 or 16");
+  RT_REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= 32, RT_ERR_RANGE, "wgs_per_cu must be in [1, 32]");
+  RT_REQUIRE(tree_depth >= 1 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [1, 22]");
+  b->tile_shape = tile_shape;
+  b->lds_levels = lds_levels;
+  b->wgs_per_cu = wgs_per_cu;
+  b->tree_depth = tree_depth;
+  return RT_OK;
+}
+
+int rt_batch_render(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image, rt_buf* codes,
+                    uint64_t* stats) {
+  const int rc = rayt::BatchLaunch(b, views, nviews, out_image, codes, stats != nullptr);
+  if (rc || !stats) return rc;
+  return rayt::BatchStats(b, stats);
 }
 
 int rt_batch_last_ms(rt_batch* b, double* ms) {
   RT_REQUIRE(b && ms, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(b->launched, RT_ERR_STATE, "nothing launched yet");
   RT_HIP_CHECK(hipEventSynchronize(b->ev1));
   float f = 0;
   RT_HIP_CHECK(hipEventElapsedTime(&f, b->ev0, b->ev1));
@@ -387,21 +503,20 @@ int rt_batch_autotune(rt_batch* b, const rt_view* views, int nviews, rt_buf* out
   RT_REQUIRE(repeats >= 1 && repeats <= 16, RT_ERR_RANGE, "repeats must be in [1, 16]");
   static const int kShapes[3] = {0, 1, 2};
   static const int kLevels[4] = {8, 10, 12, 16};
-  static const int kWgs[4] = {2, 4, 6, 8};
-  const int saved[3] = {b->tile_shape, b->lds_levels, b->wgs_per_cu};
+  static const int kWgs[4] = {8, 16, 24, 32};
   double best_t = 1e300;
-  int bc[3] = {saved[0], saved[1], saved[2]};
+  int bc[3] = {b->tile_shape, b->lds_levels, b->wgs_per_cu};
   for (int shape : kShapes)
     for (int levels : kLevels)
       for (int wgs : kWgs) {
-        // LDS per workgroup: the stack window plus the view table; skip shapes that cannot be resident.
-        const int lds_bytes = levels * 256 * 8 + (kMaxBatchViews + 1) * 4;
+        // LDS per one-wave workgroup is the stack window; skip shapes that cannot be resident.
+        const int lds_bytes = levels * 64 * 8;
         if (wgs * lds_bytes > 160 * 1024) continue;
         b->tile_shape = shape;
         b->lds_levels = levels;
         b->wgs_per_cu = wgs;
         std::vector<double> t;
-        for (int r = 0; r < repeats + 1; ++r) {  // first launch warms caches and is not counted
+        for (int r = 0; r < repeats + 1; ++r) {  // the first launch warms the caches and is not counted
           int rc = rt_batch_render(b, views, nviews, out_image, codes, nullptr);
           if (rc) return rc;
           double ms;

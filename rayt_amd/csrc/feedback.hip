@@ -231,48 +231,75 @@ __device__ __forceinline__ int WaveMax(int v) {
   return v;
 }
 
+// Marks the distinct codes in the bitmaps. Within a wave only one lane per
+// distinct value touches the bitmap (a ballot loop over the values present),
+// and the maxima / flags are reduced per workgroup before one atomic each, so
+// the few hot bitmap words and the state words see little contention.
 __global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
                                                        uint32_t* __restrict__ fault_bits, uint32_t* __restrict__ hit_bits,
                                                        MarkState* __restrict__ st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int code = i < n ? codes[i] : kNoHit;
-  if (block_size > 0 && code > 0 && code != kNoHit) code = ((((code >> 1) - 1) / block_size + 1) << 1) | (code & 1);
-  // Skip lanes repeating the previous lane's code (coherent images have long runs).
-  const int prev = __shfl_up(code, 1, 64);
-  const bool dup = ((threadIdx.x & 63) != 0) && prev == code;
+  __shared__ int red[4];  // max fault, max hit, NO_HIT seen, overflow
+  if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
   int mf = -1, mh = -1;
   bool nohit = false, over = false;
-  if (i < n && !dup) {
-    if (code == kNoHit) {
-      nohit = true;
-    } else if (code < 0) {
-      const uint32_t f = uint32_t(-(code + 1));
-      if (f >= kFaultDomain) {
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    const bool valid = i < n;
+    int code = valid ? codes[i] : kNoHit;
+    if (block_size > 0 && code > 0 && code != kNoHit) code = ((((code >> 1) - 1) / block_size + 1) << 1) | (code & 1);
+    bool mark = false;
+    if (valid) {
+      if (code == kNoHit) {
+        nohit = true;
+      } else if (code < 0) {
+        const uint32_t f = uint32_t(-(code + 1));
+        if (f >= kFaultDomain) {
+          over = true;
+        } else {
+          mark = true;
+          mf = max(mf, int(f));
+        }
+      } else if (uint32_t(code) >= kHitDomain) {
         over = true;
       } else {
-        const uint32_t bit = 1u << (f & 31u);
-        if (!(fault_bits[f >> 5] & bit)) atomicOr(&fault_bits[f >> 5], bit);
-        mf = int(f);
+        mark = true;
+        mh = max(mh, code);
       }
-    } else {
-      const uint32_t h = uint32_t(code);
-      if (h >= kHitDomain) {
-        over = true;
-      } else {
-        const uint32_t bit = 1u << (h & 31u);
-        if (!(hit_bits[h >> 5] & bit)) atomicOr(&hit_bits[h >> 5], bit);
-        mh = int(h);
+    }
+    uint64_t pending = __ballot(mark);
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int v = __shfl(code, leader, 64);
+      pending &= ~__ballot(mark && code == v);
+      if (lane == leader) {
+        if (v < 0) {
+          const uint32_t f = uint32_t(-(v + 1));
+          const uint32_t bit = 1u << (f & 31u);
+          if (!(fault_bits[f >> 5] & bit)) atomicOr(&fault_bits[f >> 5], bit);
+        } else {
+          const uint32_t bit = 1u << (uint32_t(v) & 31u);
+          if (!(hit_bits[uint32_t(v) >> 5] & bit)) atomicOr(&hit_bits[uint32_t(v) >> 5], bit);
+        }
       }
     }
   }
   mf = WaveMax(mf);
   mh = WaveMax(mh);
   const bool any_nohit = __any(nohit), any_over = __any(over);
-  if ((threadIdx.x & 63) == 0) {
-    if (mf >= 0) atomicMax(&st->max_fault, mf);
-    if (mh >= 0) atomicMax(&st->max_hit, mh);
-    if (any_nohit) atomicOr(&st->no_hit, 1);
-    if (any_over) atomicOr(&st->overflow, 1);
+  if (lane == 0) {
+    if (mf >= 0) atomicMax(&red[0], mf);
+    if (mh >= 0) atomicMax(&red[1], mh);
+    if (any_nohit) red[2] = 1;
+    if (any_over) red[3] = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (red[0] >= 0) atomicMax(&st->max_fault, red[0]);
+    if (red[1] >= 0) atomicMax(&st->max_hit, red[1]);
+    if (red[2]) atomicOr(&st->no_hit, 1);
+    if (red[3]) atomicOr(&st->overflow, 1);
   }
 }
 
@@ -443,7 +470,8 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   }
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
-  MarkCodesKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_, st);
+  MarkCodesKernel<<<dim3(std::min(Grid(count_, 256), std::max(1, ctx_->compute_units) * 8)), dim3(256), 0, s>>>(
+      codes, count_, bs, fault_bits_, hit_bits_, st);
   WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_);
   RT_HIP_CHECK(hipGetLastError());
   RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));

@@ -13,7 +13,8 @@ struct rt_ctx {
   int compute_units = 0;
   hipStream_t stream = nullptr;
   int plain_lds_levels = 16;  // LDS stack levels for the plain kernel
-  int plain_tile = 0;         // 0: 16x16 workgroup of 8x8 wave tiles
+  int plain_waves = 4;        // waves per workgroup of the plain kernel (1: 8x8 pixels, 4: 16x16)
+  int plain_lds_base = 0;     // first stack level held in LDS
 };
 
 struct rt_buf {

@@ -155,26 +155,32 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 }
 
 // ---------------------------------------------------------------- plain fused
-template <int LDS_LEVELS, bool COUNT>
-__global__ __launch_bounds__(256) void RenderPlainKernel(uint32_t* __restrict__ out, int* __restrict__ codes, int W,
-                                                         int H, PoolView pool, uint32_t root,
-                                                         const uint32_t* __restrict__ normals,
-                                                         const uint32_t* __restrict__ colors, int nodes_in_block,
-                                                         Mat16 m, float lod, Vec4 bg, int flags, Lights lt,
-                                                         unsigned long long* __restrict__ stats) {
-  __shared__ uint2 lds[LDS_LEVELS * 256];
-  const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
-  int x, y;
-  TilePixel(XcdTile(blockIdx.x, tiles), tiles_x, x, y);
+// WAVES = 4: workgroups of 256 lanes over 16x16 pixels (four 8x8 wave tiles);
+// WAVES = 1: one wave per workgroup over an 8x8 tile, so a finished wave frees
+// its LDS stack at once and more waves fit a CU.
+template <int WAVES, int LDS_LEVELS, bool COUNT>
+__global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
+    uint32_t* __restrict__ out, int* __restrict__ codes, int W, int H, PoolView pool, uint32_t root,
+    const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors, int nodes_in_block, Mat16 m,
+    float lod, Vec4 bg, int flags, Lights lt, int lds_base, unsigned long long* __restrict__ stats) {
+  constexpr int BLOCK = 64 * WAVES;
+  constexpr int TS = WAVES == 4 ? 16 : 8;
+  __shared__ uint2 lds[LDS_LEVELS * BLOCK];
+  const uint32_t tiles_x = (W + TS - 1) / TS, tiles = tiles_x * ((H + TS - 1) / TS);
+  const uint32_t tile = XcdTile(blockIdx.x, tiles);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const int x = int((tile % tiles_x) * TS + (wave & 1u) * 8u + (lane & 7u));
+  const int y = int((tile / tiles_x) * TS + (wave >> 1) * 8u + (lane >> 3));
   PixelCounters cnt;
   if (x < W && y < H) {
     const int64_t i = int64_t(y) * W + x;
-    SplitStack<LDS_LEVELS, 256> stack;
+    SplitStack<LDS_LEVELS, BLOCK> stack;
     stack.lds = lds + threadIdx.x;
+    stack.base = lds_base;
     uint32_t color;
     int code;
-    RenderPixel<SplitStack<LDS_LEVELS, 256>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y, W, H,
-                                                    lod, bg.v, flags, lt, stack, color, code, cnt);
+    RenderPixel<SplitStack<LDS_LEVELS, BLOCK>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y, W, H,
+                                                      lod, bg.v, flags, lt, stack, color, code, cnt);
     out[i] = color;
     if (codes) codes[i] = code;
   }
@@ -313,27 +319,30 @@ void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, in
                                                   lt);
 }
 
-template <int L, bool C>
+template <int WV, int L, bool C>
 void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
-                  uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
-                  const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, unsigned long long* st) {
-  RenderPlainKernel<L, C><<<g, dim3(256), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg,
-                                                  flags, lt, st);
+                  uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib, const Mat16& m, float lod,
+                  const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st) {
+  RenderPlainKernel<WV, L, C><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg,
+                                                          flags, lt, base, st);
 }
 
 template <bool C>
-bool DispatchPlain(int levels, dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H,
-                   const PoolView& pool, uint32_t root, const uint32_t* nrm,
-                   const uint32_t* col, int nib, const Mat16& m, float lod, const Vec4& bg, int flags,
-                   const Lights& lt, unsigned long long* st) {
-  switch (levels) {
-#define RT_CASE(L) \
-  case L: LaunchPlainL<L, C>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, st); return true;
-    RT_CASE(8) RT_CASE(10) RT_CASE(12) RT_CASE(14) RT_CASE(16) RT_CASE(22)
-#undef RT_CASE
-    default:
-      return false;
+bool DispatchPlain(int waves, int levels, hipStream_t s, uint32_t* out, int* codes, int W, int H,
+                   const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
+                   const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, int base,
+                   unsigned long long* st) {
+  const int ts = waves == 4 ? 16 : 8;
+  const dim3 g(uint32_t((W + ts - 1) / ts) * uint32_t((H + ts - 1) / ts));
+#define RT_CASE(WV, L)                                                                                       \
+  if (waves == WV && levels == L) {                                                                          \
+    LaunchPlainL<WV, L, C>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, base, st); \
+    return true;                                                                                             \
   }
+  RT_CASE(1, 8) RT_CASE(1, 10) RT_CASE(1, 12) RT_CASE(1, 14) RT_CASE(1, 16) RT_CASE(1, 22)
+  RT_CASE(4, 8) RT_CASE(4, 10) RT_CASE(4, 12) RT_CASE(4, 14) RT_CASE(4, 16) RT_CASE(4, 22)
+#undef RT_CASE
+  return false;
 }
 
 }  // namespace
@@ -450,14 +459,17 @@ int rt_legacy_links_from_cache(rt_ctx* ctx, rt_buf* legacy_links, const rt_buf* 
   return RT_OK;
 }
 
-int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int tile_shape) {
+int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int wg_waves, int tree_depth) {
   RT_REQUIRE(ctx, RT_ERR_INVALID, "null context");
   RT_REQUIRE(lds_stack_levels == 8 || lds_stack_levels == 10 || lds_stack_levels == 12 || lds_stack_levels == 14 ||
                  lds_stack_levels == 16 || lds_stack_levels == 22,
              RT_ERR_RANGE, "lds_stack_levels must be one of 8,10,12,14,16,22");
-  RT_REQUIRE(tile_shape == 0, RT_ERR_RANGE, "tile_shape must be 0 (8x8 wave tiles)");
+  RT_REQUIRE(wg_waves == 1 || wg_waves == 4, RT_ERR_RANGE, "wg_waves must be 1 or 4");
+  RT_REQUIRE(tree_depth >= 0 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [0, 22]");
   ctx->plain_lds_levels = lds_stack_levels;
-  ctx->plain_tile = tile_shape;
+  ctx->plain_waves = wg_waves;
+  // A depth-D tree pushes stack levels 0 .. D-2; keep the deepest ones in LDS.
+  ctx->plain_lds_base = tree_depth > 0 ? std::max(0, (tree_depth - 1) - lds_stack_levels) : 0;
   return RT_OK;
 }
 
@@ -478,16 +490,15 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
   Vec4 bg;
   memcpy(bg.v, background, sizeof(bg.v));
   const Lights lt = rayt::MakeLights();
-  const dim3 grid(TileGrid(width, height));
   if (flags & RT_FLAG_COUNT) {
     RT_REQUIRE(stats, RT_ERR_INVALID, "counting render needs a stats array");
     unsigned long long* d_stats = nullptr;
     RT_HIP_CHECK(hipMalloc(&d_stats, 4 * sizeof(unsigned long long)));
     hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream);
-    DispatchPlain<true>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+    DispatchPlain<true>(ctx->plain_waves, ctx->plain_lds_levels, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
                         codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                         uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
-                        lod_voxel_size, bg, flags, lt, d_stats);
+                        lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, d_stats);
     hipError_t e = hipGetLastError();
     unsigned long long h[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
@@ -497,10 +508,10 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
     for (int k = 0; k < 4; ++k) stats[k] = h[k];
     return RT_OK;
   }
-  DispatchPlain<false>(ctx->plain_lds_levels, grid, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+  DispatchPlain<false>(ctx->plain_waves, ctx->plain_lds_levels, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                        uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
-                       lod_voxel_size, bg, flags, lt, nullptr);
+                       lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
 }

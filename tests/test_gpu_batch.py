@@ -45,7 +45,7 @@ def test_batch_matches_plain(gpu_ctx, scene, tile_shape, lds_levels):
     specs = _views(12, seed=tile_shape * 10 + lds_levels)
     flags = api.L.RT_FLAG_SHADOWS
     b = api.BatchRenderer(m, 64)
-    b.config(tile_shape=tile_shape, lds_levels=lds_levels, wgs_per_cu=2, tree_depth=9)
+    b.config(tile_shape=tile_shape, lds_levels=lds_levels, wgs_per_cu=8, tree_depth=9)
     views, total, offsets = b.make_views(specs, flags=flags)
     out = api.Buffer(gpu_ctx, 4 * total)
     codes = api.Buffer(gpu_ctx, 4 * total)
@@ -62,7 +62,7 @@ def test_batch_counts_add_up(gpu_ctx, scene):
     m = scene
     specs = _views(9, seed=3)
     b = api.BatchRenderer(m, 16)
-    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=4, tree_depth=9)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=9)
     views, total, _ = b.make_views(specs, flags=api.L.RT_FLAG_SHADOWS)
     out = api.Buffer(gpu_ctx, 4 * total)
     st = b.render(views, len(specs), out, None, count=True)
@@ -79,7 +79,7 @@ def test_batch_repeatable_and_single_view(gpu_ctx, scene):
     m = scene
     vpi = camera_vpi(*POSES[0], aspect=640 / 480)
     b = api.BatchRenderer(m, 4)
-    b.config(tile_shape=1, lds_levels=10, wgs_per_cu=8, tree_depth=9)
+    b.config(tile_shape=1, lds_levels=10, wgs_per_cu=32, tree_depth=9)
     views, total, _ = b.make_views([(vpi, 640, 480)])
     out = api.Buffer(gpu_ctx, 4 * total)
     b.render(views, 1, out)

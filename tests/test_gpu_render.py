@@ -85,6 +85,27 @@ def test_render_plain_matches_oracle(gpu_ctx, lod, flags):
         np.testing.assert_array_equal(img, r["image"], err_msg=f"image differs at pose {yaw, pitch, pos}")
 
 
+@pytest.mark.parametrize("waves,levels,depth", [(1, 8, 8), (1, 10, 0), (4, 8, 8), (1, 22, 8), (4, 14, 8)])
+def test_render_plain_launch_shapes(gpu_ctx, waves, levels, depth):
+    """Every launch shape (workgroup size, LDS stack window, spill to private memory) gives the same bytes."""
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    W, H = 200, 120
+    try:
+        api.plain_config(gpu_ctx, levels, waves, depth)
+        for yaw, pitch, pos in POSES[:3]:
+            vpi = camera_vpi(yaw, pitch, pos)
+            out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi, flags=api.L.RT_FLAG_SHADOWS)
+            r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi,
+                                 flags=api.L.RT_FLAG_SHADOWS)
+            np.testing.assert_array_equal(out.read_all(np.uint32).reshape(H, W), r["image"])
+            np.testing.assert_array_equal(codes.read_all(np.int32).reshape(H, W), r["codes"])
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+
+
 def test_render_counts_match_oracle(gpu_ctx):
     """The counting kernel's totals (rays, shadow rays, fetches F, hits) equal the oracle's."""
     tree, path = terrain_tree(8, 8192, 8, 0)

@@ -1,0 +1,180 @@
+"""GPU tests of the continuous-batching server: every request's image and
+feedback codes are the same alone, in a mix of scenes and sizes, under a
+framebuffer arena too small for the queue, and with a block-pool budget that
+holds one scene at a time; and all of them equal the plain single-view render
+(itself bit-exact against the oracle), which the first requests also meet
+directly against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from rayt_amd import api
+from tests.scenes import POSES, camera_vpi, terrain_tree
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(320, 240), (256, 256), (100, 37), (640, 480), (7, 5), (200, 150)]
+SHADOWS = api.L.RT_FLAG_SHADOWS
+
+
+def _pool_bytes(tree):
+    i = tree.info()
+    return i.blocks_count * (i.bytes_in_block_content + 36)
+
+
+@pytest.fixture(scope="module")
+def scenes(gpu_ctx):
+    out = []
+    for depth, nib, spheres, boxes in [(8, 8192, 8, 0), (9, 1024, 0, 16)]:
+        tree, path = terrain_tree(depth, nib, spheres, boxes)
+        m = api.GPUOctreeCacheManager(tree.info().blocks_count, tree, gpu_ctx)
+        m.InitialFillCache()
+        out.append((tree, path, m, depth))
+    return out
+
+
+def _requests(n, nscenes, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        w, h = SIZES[i % len(SIZES)]
+        if i < len(POSES):
+            yaw, pitch, pos = POSES[i]
+        else:
+            yaw, pitch = float(rng.uniform(-180, 180)), float(rng.uniform(-60, 30))
+            pos = tuple(float(x) for x in rng.uniform(0, 1, 3))
+        reqs.append((camera_vpi(yaw, pitch, pos, aspect=w / h), w, h, i % nscenes))
+    return reqs
+
+
+def _plain(gpu_ctx, scenes, req):
+    vpi, w, h, s = req
+    out, codes, _ = api.render_plain(gpu_ctx, scenes[s][2], w, h, vpi, flags=SHADOWS)
+    return out.read_all(np.uint32), codes.read_all(np.int32)
+
+
+def _serve(server, reqs, one_at_a_time=False):
+    results = {}
+    if one_at_a_time:
+        ids = []
+        for vpi, w, h, s in reqs:
+            ids.append(server.submit(vpi, w, h, scene=s, flags=SHADOWS, want_codes=True))
+            server.drain()
+    else:  # one burst: the server sees the whole queue at once
+        ids = server.submit_many(reqs, flags=SHADOWS, want_codes=True)
+    server.drain()
+    got = server.poll(cap=4096)
+    for c, img, cod in got:
+        assert c.status == 0, f"request {c.id} failed with {c.status}"
+        assert c.submit_ms <= c.admit_ms <= c.done_ms
+        results[c.id] = (img, cod, c.round_views)
+    assert sorted(results) == sorted(ids)
+    return [results[i] for i in ids]
+
+
+def _server(gpu_ctx, scenes, **kw):
+    srv = api.RenderServer(gpu_ctx, **kw)
+    for tree, _, _, depth in scenes:
+        srv.add_scene(tree, depth)
+    return srv
+
+
+def test_server_alone_equals_mix_equals_plain(gpu_ctx, scenes):
+    reqs = _requests(24, len(scenes), seed=11)
+    srv = _server(gpu_ctx, scenes, max_views_per_round=16, rounds_in_flight=2, framebuffer_bytes=64 << 20)
+    try:
+        alone = _serve(srv, reqs, one_at_a_time=True)
+        assert all(rv == 1 for _, _, rv in alone)
+        srv.reset_stats()
+        mix = _serve(srv, reqs)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st.completed == len(reqs) and st.failed == 0
+    assert st.rounds == 2, "a burst of 24 requests fills rounds of up to 16 views"
+    assert max(rv for _, _, rv in mix) == 16
+    for i, req in enumerate(reqs):
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(alone[i][0], mix[i][0], err_msg=f"request {i} image alone vs mix")
+        np.testing.assert_array_equal(alone[i][1], mix[i][1], err_msg=f"request {i} codes alone vs mix")
+        np.testing.assert_array_equal(mix[i][0], pi, err_msg=f"request {i} image vs plain")
+        np.testing.assert_array_equal(mix[i][1], pc, err_msg=f"request {i} codes vs plain")
+    # the first requests of each scene directly against the oracle
+    for i in range(4):
+        vpi, w, h, s = reqs[i]
+        _, path, _, _ = scenes[s]
+        t = orc.Tree(path)
+        c = orc.Cache(t, t.blocks)
+        c.initial_fill()
+        r = orc.render_plain(c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
+                             c.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block, w, h, vpi, flags=SHADOWS)
+        np.testing.assert_array_equal(mix[i][0], r["image"].reshape(-1))
+        np.testing.assert_array_equal(mix[i][1], r["codes"].reshape(-1))
+    assert st.primary_rays == st.pixels == sum(w * h for _, w, h, _ in reqs)
+    assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
+
+
+def test_server_admission_by_framebuffer(gpu_ctx, scenes):
+    """An arena that holds two 640x480 outputs still serves a queue of ten; bigger requests are refused."""
+    reqs = [(camera_vpi(*POSES[i % len(POSES)], aspect=4 / 3), 640, 480, i % 2) for i in range(10)]
+    srv = _server(gpu_ctx, scenes, max_views_per_round=8, rounds_in_flight=2, framebuffer_bytes=8 * 640 * 480 * 2)
+    try:
+        with pytest.raises(api.RaytError):
+            srv.submit(reqs[0][0], 1280, 720, scene=0)
+        res = _serve(srv, reqs)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st.completed == 10 and st.mean_views_per_round == 2.0
+    for i, req in enumerate(reqs):
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(res[i][0], pi)
+        np.testing.assert_array_equal(res[i][1], pc)
+
+
+def test_server_admission_by_block_pool(gpu_ctx, scenes):
+    """A pool budget that holds one scene: scenes are loaded and evicted between rounds, results unchanged."""
+    budget = max(_pool_bytes(t) for t, _, _, _ in scenes)
+    reqs = _requests(12, len(scenes), seed=5)
+    srv = _server(gpu_ctx, scenes, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=64 << 20,
+                  pool_bytes=budget)
+    try:
+        res = _serve(srv, reqs)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st.completed == 12
+    assert st.scene_loads >= 2 and st.scene_evictions >= 1
+    for i, req in enumerate(reqs):
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(res[i][0], pi)
+        np.testing.assert_array_equal(res[i][1], pc)
+
+
+def test_server_refuses_scene_over_budget(gpu_ctx, scenes):
+    tree = scenes[0][0]
+    srv = api.RenderServer(gpu_ctx, pool_bytes=_pool_bytes(tree) - 1, framebuffer_bytes=1 << 20)
+    try:
+        with pytest.raises(api.RaytError):
+            srv.add_scene(tree, 8)
+    finally:
+        srv.close()
+
+
+def test_batch_views_over_several_scenes(gpu_ctx, scenes):
+    """rt_batch_add_scene: one launch over views of two resident scenes equals their plain renders."""
+    b = api.BatchRenderer(scenes[0][2], 32)
+    assert b.add_scene(scenes[1][2]) == 1
+    b.config(tile_shape=0, lds_levels=8, wgs_per_cu=8, tree_depth=9)
+    reqs = _requests(10, 2, seed=2)
+    views, total, offsets = b.make_views(reqs, flags=SHADOWS)
+    out = api.Buffer(gpu_ctx, 4 * total)
+    codes = api.Buffer(gpu_ctx, 4 * total)
+    b.render(views, len(reqs), out, codes)
+    img, cod = out.read_all(np.uint32), codes.read_all(np.int32)
+    for req, off in zip(reqs, offsets):
+        w, h = req[1], req[2]
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(img[off:off + w * h], pi)
+        np.testing.assert_array_equal(cod[off:off + w * h], pc)
+    b.close()
