@@ -106,6 +106,112 @@ def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: 
     return res
 
 
+def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16):
+    """Poisson request stream over resident scenes through the native continuous-batching server.
+
+    scenes: list of (tree, depth). Each request is a seeded pose (position uniform in [0,1]^3, direction
+    uniform on the sphere) and a resolution from the batched mix, on a uniformly drawn scene. Returns the
+    open-loop run (p50/p99 latency at the offered rate) and a closed burst (all requests queued at once:
+    the server's throughput at saturation)."""
+    import numpy as np
+    from rayt_amd import api
+    flags = api.L.RT_FLAG_SHADOWS
+    rng = np.random.default_rng(seed)
+    srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
+                           pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=16)
+    for tree, depth in scenes:
+        srv.add_scene(tree, depth)
+
+    def make(n):
+        specs = view_mix(n, int(rng.integers(1 << 30)))
+        return [(vpi, w, h, int(rng.integers(len(scenes)))) for vpi, w, h in specs]
+
+    # warm-up: load every scene, first launches
+    srv.submit_many([(s[0], s[1], s[2], k) for k, s in enumerate(make(len(scenes)))], flags=flags)
+    srv.drain(600000)
+    srv.poll(4096)
+
+    # per-request check: served results of the first requests equal the plain single-view render
+    chk = make(check)
+    ids = srv.submit_many(chk, flags=flags, want_codes=True)
+    srv.drain(600000)
+    got = {c.id: (img, cod) for c, img, cod in srv.poll(4096)}
+    caches = {}
+    bad = 0
+    for rid, (vpi, w, h, s) in zip(ids, chk):
+        if s not in caches:
+            tree, _ = scenes[s]
+            caches[s] = api.GPUOctreeCacheManager(tree.info().blocks_count, tree, ctx)
+            caches[s].InitialFillCache()
+        out, codes, _ = api.render_plain(ctx, caches[s], w, h, vpi, flags=flags)
+        img, cod = got[rid]
+        if not (np.array_equal(img, out.read_all(np.uint32)) and np.array_equal(cod, codes.read_all(np.int32))):
+            bad += 1
+    del caches
+    if bad:
+        raise SystemExit(f"{bad} served requests differ from the plain render")
+
+    # Measured requests are delivered in the server's pinned memory and released as soon as their
+    # completion is seen (the client only reads them), so the host copy to a caller buffer is not timed.
+    def collect(timeout_ms=0):
+        got = srv.poll(1024, timeout_ms=timeout_ms, keep_pinned=True)
+        for c, _, _ in got:
+            if c.status != 0:
+                raise SystemExit(f"served request {c.id} failed ({c.status})")
+            srv.release(c.id)
+        return len(got)
+
+    # open loop: Poisson arrivals at `rate`
+    n = max(1, int(rate * seconds))
+    reqs = make(n)
+    gaps = rng.exponential(1.0 / rate, n)
+    srv.reset_stats()
+    t0 = time.perf_counter()
+    due = t0
+    done = 0
+    next_log = t0 + 10.0
+    for i, (vpi, w, h, s) in enumerate(reqs):
+        due += gaps[i]
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            done += collect()
+            time.sleep(min(0.0005, max(0.0, due - now)))
+        srv.submit(vpi, w, h, scene=s, flags=flags, pinned=True)
+        if time.perf_counter() > next_log:
+            log(f"served: {i + 1}/{n} submitted")
+            next_log += 10.0
+    while done < n:
+        done += collect(1000)
+    st = srv.stats()
+    wall = time.perf_counter() - t0
+    open_loop = {"rate_req_s": rate, "seconds": round(wall, 2), "requests": int(st.completed),
+                 "failed": int(st.failed), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+                 "mean_ms": round(st.mean_ms, 3), "max_ms": round(st.max_ms, 3),
+                 "mrays_s": round((st.primary_rays + st.shadow_rays) / wall / 1e6, 1),
+                 "rounds": int(st.rounds), "mean_views_per_round": round(st.mean_views_per_round, 2),
+                 "max_rounds_in_flight": int(st.max_rounds_in_flight), "scene_loads": int(st.scene_loads)}
+    log(f"served open loop: {open_loop}")
+
+    # closed burst: everything queued at once
+    reqs = make(burst)
+    srv.reset_stats()
+    srv.submit_many(reqs, flags=flags, pinned=True)
+    done = 0
+    while done < burst:
+        done += collect(1000)
+    st = srv.stats()
+    closed = {"requests": int(st.completed), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+              "mrays_s": round(st.mrays_per_s, 1), "span_ms": round(st.span_ms, 2), "rounds": int(st.rounds),
+              "mean_views_per_round": round(st.mean_views_per_round, 2),
+              "max_rounds_in_flight": int(st.max_rounds_in_flight)}
+    log(f"served burst: {closed}")
+    srv.close()
+    return {"scene_depths": [d for _, d in scenes], "pool_budget_bytes": 16 << 30, "framebuffer_bytes": 2 << 30,
+            "requests_checked_bit_exact": check, "open_loop": open_loop, "burst": closed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +223,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-burst", type=int, default=256, help="requests queued at once for the saturation run")
+    ap.add_argument("--serve-depths", default="9,10,11,12",
+                    help="depths of the resident served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run a device-to-device copy of MB MiB (3 times) as the byte-counter reference")
     args = ap.parse_args()
@@ -203,6 +315,19 @@ def main():
 
     img = tr.out_image()
     batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3))
+    served = None
+    if not args.no_served:
+        scenes = []
+        for k, d in enumerate(int(x) for x in args.serve_depths.split(",")):
+            if d == args.depth and all(sd != d for _, sd in scenes):
+                scenes.append((tree, d))
+                continue
+            t = time.time()
+            sspec = api.TerrainSpec(depth=d, seed=2 + k, octaves=4, boxes=64, box_emin=0.01, box_emax=0.1)
+            scenes.append((api.StoredOctree.terrain(sspec, 8192), d))
+            log(f"served scene depth {d} (seed {2 + k}): {scenes[-1][0].info().blocks_count} blocks, "
+                f"built in {time.time() - t:.1f}s")
+        served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst)
     import oracle as orc  # CPU baseline + one-frame check of the measured output
 
     cpu = None
@@ -253,6 +378,7 @@ def main():
         "cpu_baseline": cpu,
         "kernel_mrays_s": round(rays / avg_kernel_ms / 1e3, 1),
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

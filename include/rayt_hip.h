@@ -323,23 +323,33 @@ typedef struct rt_server_config {
   int tile_shape, lds_levels, wgs_per_cu; /* launch shape of every round (rt_batch_config) */
 } rt_server_config;
 
+#define RT_OUT_IMAGE 1      /* bring the RGBA8 image to host memory */
+#define RT_OUT_CODES 2      /* bring the feedback codes to host memory */
+#define RT_DELIVER_COPY 0   /* results are copied into out_image / out_codes before completion */
+#define RT_DELIVER_PINNED 1 /* the completion points into the server's pinned memory until rt_server_release */
+
 typedef struct rt_request {
   float view_proj_inv[16];
   int width, height;
   float lod_voxel_size;
   int flags;           /* RT_FLAG_SHADOWS */
   int scene;           /* index returned by rt_server_add_scene */
-  uint32_t* out_image; /* host, width*height RGBA8 words, or NULL; must stay valid until completion */
-  int32_t* out_codes;  /* host, width*height feedback codes, or NULL */
+  int outputs;         /* RT_OUT_IMAGE | RT_OUT_CODES (0: render only) */
+  int delivery;        /* RT_DELIVER_COPY or RT_DELIVER_PINNED */
+  int pad;
+  uint32_t* out_image; /* RT_DELIVER_COPY + RT_OUT_IMAGE: width*height words, valid until completion */
+  int32_t* out_codes;  /* RT_DELIVER_COPY + RT_OUT_CODES: width*height codes, valid until completion */
 } rt_request;
 
 typedef struct rt_completion {
   uint64_t id;
-  int status;          /* RT_OK, or the error that failed the request's round */
-  int round_views;     /* views in the round that rendered it */
-  double submit_ms;    /* times on the server's clock (ms since rt_server_create) */
+  int status;              /* RT_OK, or the error that failed the request's round */
+  int round_views;         /* views in the round that rendered it */
+  double submit_ms;        /* times on the server's clock (ms since rt_server_create) */
   double admit_ms;
-  double done_ms;      /* result in host memory */
+  double done_ms;          /* result in host memory */
+  const uint32_t* image;   /* RT_DELIVER_PINNED: the image in pinned memory (NULL otherwise) */
+  const int32_t* codes;    /* RT_DELIVER_PINNED: the codes in pinned memory (NULL otherwise) */
 } rt_completion;
 
 typedef struct rt_server_stats {
@@ -367,6 +377,9 @@ int rt_server_submit(rt_server* server, const rt_request* req, uint64_t* id);
 int rt_server_submit_many(rt_server* server, const rt_request* reqs, int n, uint64_t* ids);
 /* Returns up to cap completions, waiting up to timeout_ms for the first one; the count or a negative error. */
 int rt_server_poll(rt_server* server, rt_completion* out, int cap, int timeout_ms);
+/* Returns the pinned result of a RT_DELIVER_PINNED request to the server: its framebuffer range is free
+ * for later requests, and the completion's pointers are no longer valid. */
+int rt_server_release(rt_server* server, uint64_t id);
 /* Waits until every submitted request has completed (or timeout_ms passes: RT_ERR_STATE). */
 int rt_server_drain(rt_server* server, int timeout_ms);
 int rt_server_stats_get(rt_server* server, rt_server_stats* out);

@@ -142,6 +142,7 @@ SIGNATURES = {
     "rt_server_submit_many": (I, [P, C.c_void_p, I, U64P]),
     "rt_server_poll": (I, [P, C.c_void_p, I, I]),
     "rt_server_drain": (I, [P, I]),
+    "rt_server_release": (I, [P, U64]),
     "rt_server_stats_get": (I, [P, C.c_void_p]),
     "rt_server_reset_stats": (I, [P]),
 }
@@ -157,14 +158,21 @@ class ServerConfig(C.Structure):
                 ("pool_bytes", U64), ("tile_shape", I), ("lds_levels", I), ("wgs_per_cu", I)]
 
 
+RT_OUT_IMAGE = 1
+RT_OUT_CODES = 2
+RT_DELIVER_COPY = 0
+RT_DELIVER_PINNED = 1
+
+
 class Request(C.Structure):
     _fields_ = [("view_proj_inv", F * 16), ("width", I), ("height", I), ("lod_voxel_size", F), ("flags", I),
-                ("scene", I), ("out_image", C.c_void_p), ("out_codes", C.c_void_p)]
+                ("scene", I), ("outputs", I), ("delivery", I), ("pad", I), ("out_image", C.c_void_p),
+                ("out_codes", C.c_void_p)]
 
 
 class Completion(C.Structure):
     _fields_ = [("id", U64), ("status", I), ("round_views", I), ("submit_ms", D), ("admit_ms", D),
-                ("done_ms", D)]
+                ("done_ms", D), ("image", C.c_void_p), ("codes", C.c_void_p)]
 
 
 class ServerStats(C.Structure):

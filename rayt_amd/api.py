@@ -584,40 +584,55 @@ class RenderServer:
         return idx
 
     def submit(self, vpi, width: int, height: int, scene: int = 0, flags: int = 0, lod: float = 0.0,
-               want_image: bool = True, want_codes: bool = False) -> int:
-        return self.submit_many([(vpi, width, height, scene)], flags, lod, want_image, want_codes)[0]
+               want_image: bool = True, want_codes: bool = False, pinned: bool = False) -> int:
+        return self.submit_many([(vpi, width, height, scene)], flags, lod, want_image, want_codes, pinned)[0]
 
     def submit_many(self, specs, flags: int = 0, lod: float = 0.0, want_image: bool = True,
-                    want_codes: bool = False) -> list[int]:
-        """specs: list of (view_proj_inv, width, height, scene), queued together. Returns request ids."""
+                    want_codes: bool = False, pinned: bool = False) -> list[int]:
+        """specs: list of (view_proj_inv, width, height, scene), queued together. Returns request ids.
+        pinned: results are delivered in the server's pinned memory (see poll) instead of being copied."""
         n = len(specs)
         arr = (L.Request * n)()
         bufs = []
         for i, (vpi, width, height, scene) in enumerate(specs):
-            img = np.empty(width * height, np.uint32) if want_image else None
-            cod = np.empty(width * height, np.int32) if want_codes else None
+            img = np.empty(width * height, np.uint32) if want_image and not pinned else None
+            cod = np.empty(width * height, np.int32) if want_codes and not pinned else None
             r = arr[i]
             r.view_proj_inv[:] = [float(x) for x in np.asarray(vpi, np.float32).reshape(16)]
             r.width, r.height, r.lod_voxel_size, r.flags, r.scene = width, height, lod, flags, scene
+            r.outputs = (L.RT_OUT_IMAGE if want_image else 0) | (L.RT_OUT_CODES if want_codes else 0)
+            r.delivery = L.RT_DELIVER_PINNED if pinned else L.RT_DELIVER_COPY
             r.out_image = img.ctypes.data if img is not None else None
             r.out_codes = cod.ctypes.data if cod is not None else None
-            bufs.append((img, cod))
+            bufs.append((img, cod, width * height))
         ids = (C.c_uint64 * n)()
         call("rt_server_submit_many", self.h, C.cast(arr, C.c_void_p), n, ids)
         for i in range(n):
-            self._live[ids[i]] = (None, *bufs[i])
+            self._live[ids[i]] = bufs[i]
         return list(ids)
 
-    def poll(self, cap: int = 256, timeout_ms: int = 0):
-        """Returns a list of (completion struct, image or None, codes or None)."""
+    def poll(self, cap: int = 256, timeout_ms: int = 0, keep_pinned: bool = False):
+        """Returns a list of (completion struct, image or None, codes or None). Pinned results are copied
+        out and released, unless keep_pinned: then image/codes are views of the server's pinned memory,
+        valid until release(completion.id)."""
         arr = (L.Completion * cap)()
         n = call("rt_server_poll", self.h, C.cast(arr, C.c_void_p), cap, timeout_ms)
         out = []
         for i in range(n):
             c = arr[i]
-            _, img, cod = self._live.pop(c.id, (None, None, None))
+            img, cod, npx = self._live.pop(c.id, (None, None, 0))
+            if c.image or c.codes:
+                img = np.ctypeslib.as_array(C.cast(c.image, C.POINTER(C.c_uint32)), (npx,)) if c.image else None
+                cod = np.ctypeslib.as_array(C.cast(c.codes, C.POINTER(C.c_int32)), (npx,)) if c.codes else None
+                if not keep_pinned:
+                    img = img.copy() if img is not None else None
+                    cod = cod.copy() if cod is not None else None
+                    self.release(c.id)
             out.append((c, img, cod))
         return out
+
+    def release(self, rid: int):
+        call("rt_server_release", self.h, rid)
 
     def drain(self, timeout_ms: int = 60000):
         call("rt_server_drain", self.h, timeout_ms)

@@ -65,7 +65,9 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
                                                   unsigned long long* __restrict__ stats) {
   constexpr int TH = 64 / TW;
   __shared__ uint2 lds[LDS_LEVELS * 64];
+  StackSpill<LDS_LEVELS> spill;
   SplitStack<LDS_LEVELS, 64> stack;
+  stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
@@ -165,8 +167,10 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
       continue;  // every claimed pixel so far was outside its view; claim more
     }
     if (!active) continue;
+    const int status = TravStep<SplitStack<LDS_LEVELS, 64>, COUNT>(tr, pool, stack, fetches);
+    if (status == kRunning) continue;
     CastResult res;
-    if (!TravStep<SplitStack<LDS_LEVELS, 64>, COUNT>(tr, pool, stack, res, fetches)) continue;
+    TravResult(tr, status, res);
     uint32_t color;
     if (phase == 0) {
       if (res.error_code) {
@@ -311,7 +315,7 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
   RT_REQUIRE(ctx && out, RT_ERR_INVALID, "null argument");
   RT_REQUIRE(max_views > 0 && max_views <= (1 << 16), RT_ERR_RANGE, "max_views must be in [1, 65536]");
   rt_batch* b = new rt_batch();
-  b->ctx = ctx;
+  b->ctx = CtxRetain(ctx);
   b->stream = ctx->stream;
   b->max_views = max_views;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(DevView) * max_views);
@@ -453,7 +457,9 @@ int rt_batch_destroy(rt_batch* b) {
   if (b->h_stats) (void)hipHostFree(b->h_stats);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  rt_ctx* ctx = b->ctx;
   delete b;
+  rayt::CtxRelease(ctx);
   return RT_OK;
 }
 

@@ -23,13 +23,17 @@ int rt_cache_create(rt_ctx* ctx, rt_tree* tree, int max_blocks_count, rt_cache**
     delete c;
     throw;
   }
+  rayt::CtxRetain(ctx);
   *out = c;
   return RT_OK;
   RT_TRY_END
 }
 
 int rt_cache_destroy(rt_cache* cache) {
+  if (!cache) return RT_OK;
+  rt_ctx* ctx = cache->ctx;
   delete cache;
+  rayt::CtxRelease(ctx);
   return RT_OK;
 }
 

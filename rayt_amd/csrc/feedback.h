@@ -54,6 +54,8 @@ class FeedbackExtractor {
   int* unique_ = nullptr;
   int* temp_ = nullptr;
   int* index_ = nullptr;
+  void* partials_ = nullptr;  // int4 per marking workgroup: max fault, max hit, NO_HIT seen, overflow
+  int max_mark_blocks_ = 1;
   std::vector<int> host_sorted_, window_;
 
   void AddHit(int encoded) {

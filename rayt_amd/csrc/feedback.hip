@@ -233,11 +233,12 @@ __device__ __forceinline__ int WaveMax(int v) {
 
 // Marks the distinct codes in the bitmaps. Within a wave only one lane per
 // distinct value touches the bitmap (a ballot loop over the values present),
-// and the maxima / flags are reduced per workgroup before one atomic each, so
-// the few hot bitmap words and the state words see little contention.
+// and a bitmap word is read before it is or-ed, so the few hot words see
+// atomics only until their bits are set. The maxima / flags of each workgroup
+// go to its own slot of `partials` (no shared counter), reduced by the walk.
 __global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
                                                        uint32_t* __restrict__ fault_bits, uint32_t* __restrict__ hit_bits,
-                                                       MarkState* __restrict__ st) {
+                                                       int4* __restrict__ partials) {
   __shared__ int red[4];  // max fault, max hit, NO_HIT seen, overflow
   if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
   __syncthreads();
@@ -277,10 +278,12 @@ __global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ c
         if (v < 0) {
           const uint32_t f = uint32_t(-(v + 1));
           const uint32_t bit = 1u << (f & 31u);
-          if (!(fault_bits[f >> 5] & bit)) atomicOr(&fault_bits[f >> 5], bit);
+          uint32_t* w = &fault_bits[f >> 5];
+          if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
         } else {
           const uint32_t bit = 1u << (uint32_t(v) & 31u);
-          if (!(hit_bits[uint32_t(v) >> 5] & bit)) atomicOr(&hit_bits[uint32_t(v) >> 5], bit);
+          uint32_t* w = &hit_bits[uint32_t(v) >> 5];
+          if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
         }
       }
     }
@@ -295,22 +298,43 @@ __global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ c
     if (any_over) red[3] = 1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (red[0] >= 0) atomicMax(&st->max_fault, red[0]);
-    if (red[1] >= 0) atomicMax(&st->max_hit, red[1]);
-    if (red[2]) atomicOr(&st->no_hit, 1);
-    if (red[3]) atomicOr(&st->overflow, 1);
-  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = make_int4(red[0], red[1], red[2], red[3]);
 }
 
-// One workgroup walks the bitmaps in ascending code order and writes the
-// distinct codes, then clears what it read and resets the state.
+// One workgroup reduces the marking partials, walks the bitmaps in ascending
+// code order and writes the distinct codes, then clears what it read and
+// leaves the totals in the state.
 __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
                                                           uint32_t* __restrict__ hit_bits, MarkState* __restrict__ st,
-                                                          int* __restrict__ unique_out) {
+                                                          int* __restrict__ unique_out,
+                                                          const int4* __restrict__ partials, int nparts) {
   __shared__ uint32_t waves[1024 / 64 + 1];
-  const int wf = st->max_fault >= 0 ? (st->max_fault >> 5) + 1 : 0;
-  const int wh = st->max_hit >= 0 ? (st->max_hit >> 5) + 1 : 0;
+  __shared__ int red[4];
+  if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
+  __syncthreads();
+  {
+    int4 acc = make_int4(-1, -1, 0, 0);
+    for (int j = threadIdx.x; j < nparts; j += 1024) {
+      const int4 p = partials[j];
+      acc.x = max(acc.x, p.x);
+      acc.y = max(acc.y, p.y);
+      acc.z |= p.z;
+      acc.w |= p.w;
+    }
+    acc.x = WaveMax(acc.x);
+    acc.y = WaveMax(acc.y);
+    const bool z = __any(acc.z != 0), w = __any(acc.w != 0);
+    if ((threadIdx.x & 63) == 0) {
+      atomicMax(&red[0], acc.x);
+      atomicMax(&red[1], acc.y);
+      if (z) red[2] = 1;
+      if (w) red[3] = 1;
+    }
+  }
+  __syncthreads();
+  const int max_fault = red[0], max_hit = red[1], no_hit = red[2], overflow = red[3];
+  const int wf = max_fault >= 0 ? (max_fault >> 5) + 1 : 0;
+  const int wh = max_hit >= 0 ? (max_hit >> 5) + 1 : 0;
   const int L = wf + wh;
   const int per = (L + 1023) / 1024;
   const int b = threadIdx.x * per, e = min(L, b + per);
@@ -348,12 +372,13 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
   __syncthreads();
   if (threadIdx.x == 0) {
     int u = int(total);
-    if (st->no_hit) unique_out[u++] = kNoHit;
+    if (no_hit) unique_out[u++] = kNoHit;
     st->unique_count = u;
     st->fault_count = int(faults_total);
-    st->max_fault = -1;
-    st->max_hit = -1;
-    st->no_hit = 0;
+    st->max_fault = max_fault;
+    st->max_hit = max_hit;
+    st->no_hit = no_hit;
+    st->overflow = overflow;
   }
 }
 
@@ -397,7 +422,7 @@ int ScanI32(hipStream_t s, int* data, int n) {
 }
 
 // --------------------------------------------------------- FeedbackExtractor
-FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(ctx), count_(count) {
+FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(CtxRetain(ctx)), count_(count) {
   auto alloc = [&](void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -408,6 +433,8 @@ FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(ctx), count_
   alloc(reinterpret_cast<void**>(&unique_), size_t(std::max(count, 1) + 1) * 4);
   alloc(reinterpret_cast<void**>(&temp_), size_t(std::max(count, 1)) * 4);
   alloc(reinterpret_cast<void**>(&index_), size_t(std::max(count, 1)) * 4);
+  max_mark_blocks_ = std::max(1, ctx->compute_units) * 8;
+  alloc(&partials_, size_t(max_mark_blocks_) * sizeof(int4));
   (void)hipMemsetAsync(fault_bits_, 0, kFaultDomain / 8, ctx_->stream);
   (void)hipMemsetAsync(hit_bits_, 0, kHitDomain / 8, ctx_->stream);
   MarkState init{-1, -1, 0, 0, 0, 0, {0, 0}};
@@ -426,7 +453,9 @@ FeedbackExtractor::~FeedbackExtractor() {
   (void)hipFree(unique_);
   (void)hipFree(temp_);
   (void)hipFree(index_);
+  (void)hipFree(partials_);
   (void)hipHostFree(host_state_);
+  CtxRelease(ctx_);
 }
 
 uint32_t FeedbackExtractor::NextRandom() {
@@ -470,16 +499,16 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   }
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
-  MarkCodesKernel<<<dim3(std::min(Grid(count_, 256), std::max(1, ctx_->compute_units) * 8)), dim3(256), 0, s>>>(
-      codes, count_, bs, fault_bits_, hit_bits_, st);
-  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_);
+  const int grid = std::min(Grid(count_, 256), max_mark_blocks_);
+  MarkCodesKernel<<<dim3(grid), dim3(256), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_,
+                                                   static_cast<int4*>(partials_));
+  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+                                                   static_cast<const int4*>(partials_), grid);
   RT_HIP_CHECK(hipGetLastError());
   RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipStreamSynchronize(s));
   const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
   if (hs.overflow) {
-    int zero = 0;
-    RT_HIP_CHECK(hipMemcpyAsync(&reinterpret_cast<MarkState*>(state_)->overflow, &zero, 4, hipMemcpyHostToDevice, s));
     if (!block_level) NodeToBlockKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, nodes_in_block, codes);
     int rc = UniqueBySort(codes);
     if (rc) return rc;

@@ -8,7 +8,12 @@
 
 #include "../../include/rayt_hip.h"
 
+// Reference counted: rt_ctx_destroy drops the caller's reference, and every
+// object that keeps the context (buffers, caches, tracers, batches, servers)
+// holds one more, so the stream outlives all of them whatever the order of
+// destruction.
 struct rt_ctx {
+  std::atomic<int> refs{1};
   int device = 0;
   int compute_units = 0;
   hipStream_t stream = nullptr;
@@ -34,6 +39,11 @@ namespace rayt {
 void SetError(const std::string& msg);
 int Fail(int code, const std::string& msg);
 const char* LastError();
+
+// Context references (see rt_ctx). Release destroys the stream and the
+// context with the last reference.
+rt_ctx* CtxRetain(rt_ctx* ctx);
+void CtxRelease(rt_ctx* ctx);
 
 }  // namespace rayt
 

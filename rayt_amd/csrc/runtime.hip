@@ -17,6 +17,19 @@ int Fail(int code, const std::string& msg) {
 }
 const char* LastError() { return g_last_error.c_str(); }
 
+rt_ctx* CtxRetain(rt_ctx* ctx) {
+  if (ctx) ctx->refs.fetch_add(1, std::memory_order_relaxed);
+  return ctx;
+}
+
+void CtxRelease(rt_ctx* ctx) {
+  if (!ctx || ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 }  // namespace rayt
 
 namespace {
@@ -85,11 +98,7 @@ int rt_ctx_create(int device, rt_ctx** out) {
 }
 
 int rt_ctx_destroy(rt_ctx* ctx) {
-  if (!ctx) return RT_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  hipStreamDestroy(ctx->stream);
-  delete ctx;
+  rayt::CtxRelease(ctx);
   return RT_OK;
 }
 
@@ -113,7 +122,7 @@ int rt_buf_create(rt_ctx* ctx, uint64_t bytes, rt_buf** out) {
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) return rayt::Fail(RT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   rt_buf* b = new rt_buf();
-  b->ctx = ctx;
+  b->ctx = rayt::CtxRetain(ctx);
   b->ptr = p;
   b->size = bytes;
   *out = b;
@@ -126,6 +135,7 @@ int rt_buf_destroy(rt_buf* buf) {
     hipStreamSynchronize(buf->ctx->stream);
     hipFree(buf->ptr);
   }
+  rayt::CtxRelease(buf->ctx);
   delete buf;
   return RT_OK;
 }

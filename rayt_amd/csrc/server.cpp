@@ -138,6 +138,9 @@ struct rt_server {
   uint64_t next_id = 1;
   uint64_t submitted = 0, retired = 0;
   bool stop = false;
+  // pinned results held for their callers (id -> arena range), and ranges released by them
+  std::map<uint64_t, std::pair<uint64_t, uint64_t>> held;
+  std::vector<std::pair<uint64_t, uint64_t>> released;
   // statistics (guarded by mu)
   std::vector<double> latencies;
   rt_server_stats st{};
@@ -255,10 +258,10 @@ int rt_server::Launch(Slot& s) {
   if (rc) return rc;
   const uint64_t cap = arena->capacity();
   for (const Admitted& a : s.reqs) {
-    if (a.p.req.out_image)
+    if (a.p.req.outputs & RT_OUT_IMAGE)
       RT_HIP_CHECK(hipMemcpyAsync(pinned + a.px_offset * 4, static_cast<uint8_t*>(arena_image->ptr) + a.px_offset * 4,
                                   a.px_count * 4, hipMemcpyDeviceToHost, s.stream));
-    if (a.p.req.out_codes)
+    if (a.p.req.outputs & RT_OUT_CODES)
       RT_HIP_CHECK(hipMemcpyAsync(pinned + (cap + a.px_offset) * 4,
                                   static_cast<uint8_t*>(arena_codes->ptr) + a.px_offset * 4, a.px_count * 4,
                                   hipMemcpyDeviceToHost, s.stream));
@@ -272,16 +275,27 @@ void rt_server::Retire(Slot& s, int status) {
   if (status == RT_OK) status = rayt::BatchStats(s.batch, counts);
   const uint64_t cap = arena->capacity();
   std::vector<rt_completion> done;
+  std::vector<std::pair<uint64_t, std::pair<uint64_t, uint64_t>>> hold;
   for (const Admitted& a : s.reqs) {
-    if (status == RT_OK) {
-      if (a.p.req.out_image) memcpy(a.p.req.out_image, pinned + a.px_offset * 4, a.px_count * 4);
-      if (a.p.req.out_codes) memcpy(a.p.req.out_codes, pinned + (cap + a.px_offset) * 4, a.px_count * 4);
+    const rt_request& r = a.p.req;
+    const uint32_t* img = reinterpret_cast<const uint32_t*>(pinned + a.px_offset * 4);
+    const int32_t* cod = reinterpret_cast<const int32_t*>(pinned + (cap + a.px_offset) * 4);
+    const bool keep = status == RT_OK && r.delivery == RT_DELIVER_PINNED;
+    if (status == RT_OK && r.delivery == RT_DELIVER_COPY) {
+      if (r.outputs & RT_OUT_IMAGE) memcpy(r.out_image, img, a.px_count * 4);
+      if (r.outputs & RT_OUT_CODES) memcpy(r.out_codes, cod, a.px_count * 4);
     }
-    arena->Release(a.px_offset, a.px_count);
-    --scenes[a.p.req.scene].in_flight;
-    done.push_back(rt_completion{a.p.id, status, int(s.reqs.size()), a.p.submit_ms, a.admit_ms, Now()});
+    if (keep)
+      hold.push_back({a.p.id, {a.px_offset, a.px_count}});
+    else
+      arena->Release(a.px_offset, a.px_count);
+    --scenes[r.scene].in_flight;
+    done.push_back(rt_completion{a.p.id, status, int(s.reqs.size()), a.p.submit_ms, a.admit_ms, Now(),
+                                 keep && (r.outputs & RT_OUT_IMAGE) ? img : nullptr,
+                                 keep && (r.outputs & RT_OUT_CODES) ? cod : nullptr});
   }
   std::lock_guard<std::mutex> g(mu);
+  for (const auto& h : hold) held[h.first] = h.second;
   for (const rt_completion& c : done) {
     completions.push_back(c);
     ++retired;
@@ -306,6 +320,15 @@ void rt_server::Loop() {
   while (true) {
     bool progressed = false;
     int busy = 0;
+    {
+      std::vector<std::pair<uint64_t, uint64_t>> ranges;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        ranges.swap(released);
+      }
+      for (const auto& r : ranges) arena->Release(r.first, r.second);
+      progressed = !ranges.empty();
+    }
     for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
@@ -323,6 +346,7 @@ void rt_server::Loop() {
       std::lock_guard<std::mutex> g(mu);
       if (stop && busy == 0) break;
     }
+    bool blocked = false;  // the queue head waits for memory that only a release or a retire frees
     for (int k = 0; k < cfg.rounds_in_flight; ++k) {
       Slot& s = slots[k];
       if (s.busy) continue;
@@ -330,7 +354,10 @@ void rt_server::Loop() {
         std::lock_guard<std::mutex> g(mu);
         if (pending.empty() || stop) break;
       }
-      if (!FormRound(s)) break;
+      if (!FormRound(s)) {
+        blocked = true;
+        break;
+      }
       const double t = Now();
       const int rc = Launch(s);
       progressed = true;
@@ -355,8 +382,11 @@ void rt_server::Loop() {
       std::unique_lock<std::mutex> lk(mu);
       if (busy > 0)
         cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else if (blocked)
+        cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !released.empty(); });
       else
-        cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !pending.empty(); });
+        cv_work.wait_for(lk, std::chrono::milliseconds(2),
+                         [&] { return stop || !pending.empty() || !released.empty(); });
     }
   }
 }
@@ -374,7 +404,7 @@ int rt_server_create(rt_ctx* ctx, const rt_server_config* cfg, rt_server** out) 
   RT_REQUIRE(cfg->pool_bytes > 0, RT_ERR_RANGE, "pool_bytes must be positive");
   RT_TRY_BEGIN
   rt_server* s = new rt_server();
-  s->ctx = ctx;
+  s->ctx = rayt::CtxRetain(ctx);
   s->cfg = *cfg;
   s->t0 = Clock::now();
   const uint64_t px = cfg->framebuffer_bytes / kBytesPerPixel;
@@ -430,7 +460,9 @@ int rt_server_destroy(rt_server* s) {
   rt_buf_destroy(s->arena_codes);
   if (s->pinned) (void)hipHostFree(s->pinned);
   delete s->arena;
+  rt_ctx* ctx = s->ctx;
   delete s;
+  rayt::CtxRelease(ctx);
   return RT_OK;
 }
 
@@ -466,6 +498,11 @@ int rt_server_submit_many(rt_server* s, const rt_request* reqs, int n, uint64_t*
                "bad request size");
     RT_REQUIRE(uint64_t(r.width) * r.height <= s->arena->capacity(), RT_ERR_RANGE,
                "request larger than the framebuffer arena");
+    RT_REQUIRE(r.delivery == RT_DELIVER_COPY || r.delivery == RT_DELIVER_PINNED, RT_ERR_INVALID, "bad delivery");
+    RT_REQUIRE((r.outputs & ~(RT_OUT_IMAGE | RT_OUT_CODES)) == 0, RT_ERR_INVALID, "bad outputs");
+    RT_REQUIRE(r.delivery != RT_DELIVER_COPY || ((!(r.outputs & RT_OUT_IMAGE) || r.out_image) &&
+                                                 (!(r.outputs & RT_OUT_CODES) || r.out_codes)),
+               RT_ERR_INVALID, "copy delivery needs a destination for every requested output");
   }
   std::lock_guard<std::mutex> g(s->mu);
   RT_REQUIRE(!s->stop, RT_ERR_STATE, "server is stopping");
@@ -496,6 +533,17 @@ int rt_server_poll(rt_server* s, rt_completion* out, int cap, int timeout_ms) {
     s->completions.pop_front();
   }
   return n;
+}
+
+int rt_server_release(rt_server* s, uint64_t id) {
+  RT_REQUIRE(s, RT_ERR_INVALID, "null server");
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->held.find(id);
+  RT_REQUIRE(it != s->held.end(), RT_ERR_INVALID, "no pinned result held for this id");
+  s->released.push_back(it->second);
+  s->held.erase(it);
+  s->cv_work.notify_one();
+  return RT_OK;
 }
 
 int rt_server_drain(rt_server* s, int timeout_ms) {

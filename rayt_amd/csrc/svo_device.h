@@ -20,7 +20,7 @@ namespace rtdev {
 
 constexpr int kSMax = 23;                  // raycast.cl:10
 constexpr float kEpsilon = 0.00000011920928955078125f;  // exp2(-S_MAX), raycast.cl:11
-constexpr int kIterLimit = 300;            // raycast.cl:95
+constexpr int kIterLimitSteps = 300;          // raycast.cl:95
 constexpr int kNoHit = 0x7fffffff;         // common.cl:14
 constexpr int kStackLevels = kSMax - 1;    // levels 0..21 (scale 22..1)
 
@@ -44,25 +44,37 @@ struct CastResult {
   int node_depth;
 };
 
-// Traversal stack indexed by level = (S_MAX-1) - scale. Levels below LDS_LEVELS
-// live in LDS (one column per thread, lanes contiguous so a wave touches
-// consecutive 8-byte words); deeper levels live in a small private array that
-// is only touched for trees deeper than LDS_LEVELS+1.
+// Traversal stack indexed by level = (S_MAX-1) - scale. LDS_LEVELS levels,
+// starting at level `base`, live in LDS (one column per thread, lanes
+// contiguous so a wave touches consecutive 8-byte words); the other levels
+// live in a private array that is only touched for trees deeper than the LDS
+// window. The private array is kept outside the struct (`spill` points to a
+// StackSpill of the kernel) so that the struct itself stays in registers and
+// the LDS accesses compile to ds_read/ds_write.
+template <int LDS_LEVELS>
+struct StackSpill {
+  uint2 v[LDS_LEVELS >= kStackLevels ? 1 : kStackLevels];
+};
+
 template <int LDS_LEVELS, int BLOCK>
 struct SplitStack {
-  uint2* lds;    // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
-  int base = 0;  // first level held in LDS; levels outside [base, base + LDS_LEVELS) use `spill`
-  uint2 spill[LDS_LEVELS >= kStackLevels ? 1 : kStackLevels];
+  uint2* lds;             // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
+  int base = 0;           // first level held in LDS
+  uint2* spill = nullptr;  // StackSpill<LDS_LEVELS>::v of this thread
   __device__ __forceinline__ void put(int scale, uint2 v) {
     const int level = (kSMax - 1) - scale;
     if constexpr (LDS_LEVELS >= kStackLevels) {
       lds[level * BLOCK] = v;
     } else {
       const int l = level - base;
-      if (unsigned(l) < unsigned(LDS_LEVELS))
+      if (unsigned(l) < unsigned(LDS_LEVELS)) {
         lds[l * BLOCK] = v;
-      else
+      } else {
         spill[level] = v;
+        // keeps the two stores apart: merged, they become one flat store
+        // through a selected pointer instead of a ds_write
+        __asm__ volatile("" ::: "memory");
+      }
     }
   }
   __device__ __forceinline__ uint2 get(int scale) const {
@@ -71,8 +83,14 @@ struct SplitStack {
       return lds[level * BLOCK];
     } else {
       const int l = level - base;
-      if (unsigned(l) < unsigned(LDS_LEVELS)) return lds[l * BLOCK];
-      return spill[level];
+      uint2 r;
+      if (unsigned(l) < unsigned(LDS_LEVELS)) {
+        r = lds[l * BLOCK];
+      } else {
+        r = spill[level];
+        __asm__ volatile("" ::: "memory");
+      }
+      return r;
     }
   }
 };
@@ -138,143 +156,124 @@ __device__ __forceinline__ void TravInit(Trav& s, uint32_t root_node, float px, 
   s.ray_size_bias = ray_size_bias;
 }
 
-// One loop iteration. Returns true when the ray has finished; `res` then holds
-// hit_t, hit_node, fault_block, hit_duplicate, error_code and node_depth.
+// How a traversal ended (TravStep returns kRunning while it goes on).
+enum TravStatus : int {
+  kRunning = 0,
+  kMiss = 1,       // left the octree (raycast.cl:254-255)
+  kIterLimit = 2,  // iteration limit (raycast.cl:104-110), error 1
+  kFault = 3,      // children not loaded (raycast.cl:134-141)
+  kBadLink = 4,    // far or child pointer outside the pool, or below the finest scale: error 2
+  kHit = 5,        // leaf or LOD stop (raycast.cl:159-165); `parent` then holds the hit node
+};
+
+// One loop iteration of raycast.cl:100-250. The conditions of the reference's
+// branches are evaluated as predicates and the state is updated with selects;
+// only the three node loads (the parent descriptor after a pop, a far
+// pointer, the child descriptor) and the stack accesses are guarded. Every
+// way out of the loop only sets the status, which TravResult decodes once, so
+// the loop around the step has a single exit.
 template <class Stack, bool COUNT>
-__device__ __forceinline__ bool TravStep(Trav& s, const PoolView& pool, Stack& stack, CastResult& res,
-                                         uint32_t& fetches) {
-  if (s.scale >= kSMax) {  // left the octree: miss
-    res.fault_block = -1;
-    res.error_code = 0;
-    res.hit_duplicate = 0;
-    res.hit_t = 1e30f;
-    res.hit_node = -1;
-    res.node_depth = 0;
-    return true;
-  }
-  if (++s.iter > kIterLimit) {
-    res.fault_block = -1;
-    res.hit_duplicate = 0;
-    res.error_code = 1;
-    res.hit_t = s.t_min;
-    res.hit_node = int(s.parent);
-    res.node_depth = kSMax - s.scale - 1;
-    return true;
-  }
+__device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& stack, uint32_t& fetches) {
+  if (s.scale >= kSMax) return kMiss;
+  if (++s.iter > kIterLimitSteps) return kIterLimit;
   if (s.child_descriptor == 0) {
     s.child_descriptor = pool.links[s.parent];
     if (COUNT) ++fetches;
   }
+  const uint32_t desc = s.child_descriptor;
   const float tx_corner = s.posx * s.tx_coef - s.tx_bias;
   const float ty_corner = s.posy * s.ty_coef - s.ty_bias;
   const float tz_corner = s.posz * s.tz_coef - s.tz_bias;
   const float tc_max = cl_fmin(cl_fmin(tx_corner, ty_corner), tz_corner);
 
-  const uint32_t child_shift = s.idx ^ s.octant_mask;
-  const uint32_t child_masks = 1u << child_shift;
-  if ((s.child_descriptor & child_masks) != 0 && s.t_min <= s.t_max) {
-    if (s.child_descriptor & (1u << 8)) {  // children not loaded: fault
-      res.hit_duplicate = 0;
-      res.error_code = 0;
-      res.hit_node = int(s.parent);
-      res.hit_t = s.t_min;
-      res.fault_block = int(s.child_descriptor >> 11);
-      res.node_depth = kSMax - s.scale - 1;
-      return true;
-    }
-    uint32_t ofs;
-    if (s.child_descriptor & (1u << 10)) {
-      if ((s.child_descriptor >> 11) >= pool.far_count) {
-        res.fault_block = -1;
-        res.hit_duplicate = 0;
-        res.error_code = 2;
-        res.hit_t = s.t_min;
-        res.hit_node = int(s.parent);
-        res.node_depth = kSMax - s.scale - 1;
-        return true;
-      }
-      ofs = pool.far_ptrs[s.child_descriptor >> 11];
-      if (COUNT) ++fetches;
-    } else {
-      ofs = (s.child_descriptor >> 11) + s.parent - (1u << 20);
-    }
-    ofs += __popc(s.child_descriptor & (child_masks - 1u) & 255u);
-    if (ofs >= pool.nodes || s.scale < 1) {
-      res.fault_block = -1;
-      res.hit_duplicate = 0;
-      res.error_code = 2;
-      res.hit_t = s.t_min;
-      res.hit_node = int(s.parent);
-      res.node_depth = kSMax - s.scale - 1;
-      return true;
-    }
-    s.child_descriptor = pool.links[ofs];
+  const uint32_t child_masks = 1u << (s.idx ^ s.octant_mask);
+  const bool want = (desc & child_masks) != 0 && s.t_min <= s.t_max;
+  const bool fault = want && (desc & (1u << 8)) != 0;
+  const uint32_t ptr = desc >> 11;
+  const bool is_far = want && !fault && (desc & (1u << 10)) != 0;
+  const bool far_bad = is_far && ptr >= pool.far_count;
+  uint32_t ofs = ptr + s.parent - (1u << 20);
+  if (is_far && !far_bad) {
+    ofs = pool.far_ptrs[ptr];
     if (COUNT) ++fetches;
+  }
+  ofs += __popc(desc & (child_masks - 1u) & 255u);
+  const bool bad = far_bad || (want && !fault && (ofs >= pool.nodes || s.scale < 1));
+  const bool fetch = want && !fault && !bad;
+  uint32_t child = desc;
+  if (fetch) {
+    child = pool.links[ofs];
+    if (COUNT) ++fetches;
+  }
+  const bool hit = fetch && (!(child & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2);
+  const bool push = fetch && !hit;
+  const int status = fault ? int(kFault) : bad ? int(kBadLink) : hit ? int(kHit) : int(kRunning);
+  s.child_descriptor = child;
+  if (hit) s.parent = ofs;
 
-    if (!(s.child_descriptor & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2) {
-      res.fault_block = -1;
-      res.error_code = 0;
-      res.hit_t = s.t_min;
-      res.hit_node = int(ofs);
-      res.hit_duplicate = int((s.child_descriptor >> 9) & 1u);
-      res.node_depth = kSMax - s.scale;
-      return true;
-    }
-
+  if (push) {
     const float half = s.scale_exp2 * 0.5f;
     const float tx_center = half * s.tx_coef + tx_corner;
     const float ty_center = half * s.ty_coef + ty_corner;
     const float tz_center = half * s.tz_coef + tz_corner;
-
     if (tc_max < s.h) stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)));
     s.h = tc_max;
     s.parent = ofs;
-
-    s.idx = 0;
     s.scale--;
     s.scale_exp2 = half;
-    if (tx_center > s.t_min) s.idx ^= 1u, s.posx += s.scale_exp2;
-    if (ty_center > s.t_min) s.idx ^= 2u, s.posy += s.scale_exp2;
-    if (tz_center > s.t_min) s.idx ^= 4u, s.posz += s.scale_exp2;
+    const bool cx = tx_center > s.t_min, cy = ty_center > s.t_min, cz = tz_center > s.t_min;
+    s.idx = (cx ? 1u : 0u) | (cy ? 2u : 0u) | (cz ? 4u : 0u);
+    s.posx = cx ? s.posx + half : s.posx;
+    s.posy = cy ? s.posy + half : s.posy;
+    s.posz = cz ? s.posz + half : s.posz;
     s.t_max = tc_max;
-    return false;
+  } else if (!want) {
+    // advance
+    const bool sx = tx_corner <= tc_max, sy = ty_corner <= tc_max, sz = tz_corner <= tc_max;
+    const uint32_t step_mask = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
+    s.posx = sx ? s.posx - s.scale_exp2 : s.posx;
+    s.posy = sy ? s.posy - s.scale_exp2 : s.posy;
+    s.posz = sz ? s.posz - s.scale_exp2 : s.posz;
+    s.t_min = tc_max;
+    s.idx ^= step_mask;
+    if ((s.idx & step_mask) != 0) {
+      // pop. Highest differing bit of the stepped coordinates: the reference
+      // reads it from the exponent of (float)differing_bits; the two agree
+      // whenever the result is below S_MAX (the value is then exact in fp32),
+      // and both leave the loop otherwise.
+      const uint32_t dx = sx ? __float_as_uint(s.posx) ^ __float_as_uint(s.posx + s.scale_exp2) : 0u;
+      const uint32_t dy = sy ? __float_as_uint(s.posy) ^ __float_as_uint(s.posy + s.scale_exp2) : 0u;
+      const uint32_t dz = sz ? __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2) : 0u;
+      const int scale = 31 - int(__clz(dx | dy | dz));
+      s.scale = scale;
+      if (scale < kSMax) {  // otherwise the ray left the root cube; the next step reports the miss
+        s.scale_exp2 = __uint_as_float(uint32_t(scale - kSMax + 127) << 23);
+        const uint2 e = stack.get(scale);
+        s.parent = e.x;
+        s.t_max = __uint_as_float(e.y);
+        const uint32_t shx = __float_as_uint(s.posx) >> scale;
+        const uint32_t shy = __float_as_uint(s.posy) >> scale;
+        const uint32_t shz = __float_as_uint(s.posz) >> scale;
+        s.posx = __uint_as_float(shx << scale);
+        s.posy = __uint_as_float(shy << scale);
+        s.posz = __uint_as_float(shz << scale);
+        s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
+        s.h = 0.0f;
+        s.child_descriptor = 0;
+      }
+    }
   }
+  return status;
+}
 
-  uint32_t step_mask = 0;
-  if (tx_corner <= tc_max) step_mask ^= 1u, s.posx -= s.scale_exp2;
-  if (ty_corner <= tc_max) step_mask ^= 2u, s.posy -= s.scale_exp2;
-  if (tz_corner <= tc_max) step_mask ^= 4u, s.posz -= s.scale_exp2;
-  s.t_min = tc_max;
-  s.idx ^= step_mask;
-
-  if ((s.idx & step_mask) != 0) {
-    uint32_t differing_bits = 0;
-    if (step_mask & 1u) differing_bits |= __float_as_uint(s.posx) ^ __float_as_uint(s.posx + s.scale_exp2);
-    if (step_mask & 2u) differing_bits |= __float_as_uint(s.posy) ^ __float_as_uint(s.posy + s.scale_exp2);
-    if (step_mask & 4u) differing_bits |= __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2);
-    // Highest set bit. The reference reads it from the exponent of
-    // (float)differing_bits; the two agree whenever the result is below
-    // S_MAX (the value is then exact in fp32), and both leave the loop
-    // otherwise.
-    s.scale = 31 - __clz(differing_bits);
-    if (s.scale >= kSMax) return false;  // left the root cube; the next step reports the miss
-    s.scale_exp2 = __uint_as_float(uint32_t(s.scale - kSMax + 127) << 23);
-
-    const uint2 e = stack.get(s.scale);
-    s.parent = e.x;
-    s.t_max = __uint_as_float(e.y);
-
-    const uint32_t shx = __float_as_uint(s.posx) >> s.scale;
-    const uint32_t shy = __float_as_uint(s.posy) >> s.scale;
-    const uint32_t shz = __float_as_uint(s.posz) >> s.scale;
-    s.posx = __uint_as_float(shx << s.scale);
-    s.posy = __uint_as_float(shy << s.scale);
-    s.posz = __uint_as_float(shz << s.scale);
-    s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
-    s.h = 0.0f;
-    s.child_descriptor = 0;
-  }
-  return false;
+// The RayCastResult of a finished traversal (raycast.cl:104-165, 254-255).
+__device__ __forceinline__ void TravResult(const Trav& s, int status, CastResult& res) {
+  res.fault_block = status == kFault ? int(s.child_descriptor >> 11) : -1;
+  res.error_code = status == kIterLimit ? 1 : status == kBadLink ? 2 : 0;
+  res.hit_duplicate = status == kHit ? int((s.child_descriptor >> 9) & 1u) : 0;
+  res.hit_t = status == kMiss ? 1e30f : s.t_min;
+  res.hit_node = status == kMiss ? -1 : int(s.parent);
+  res.node_depth = status == kMiss ? 0 : status == kHit ? kSMax - s.scale : kSMax - s.scale - 1;
 }
 
 template <class Stack, bool COUNT>
@@ -283,8 +282,11 @@ __device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node
                                         Stack& stack, CastResult& res, uint32_t& fetches) {
   Trav s;
   TravInit(s, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
-  while (!TravStep<Stack, COUNT>(s, pool, stack, res, fetches)) {
-  }
+  int status;
+  do {
+    status = TravStep<Stack, COUNT>(s, pool, stack, fetches);
+  } while (status == kRunning);
+  TravResult(s, status, res);
 }
 
 // mul(float16, float3) of init_tracing_frame.cl:3-9: (x,y,z,1) against rows,

@@ -99,7 +99,9 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
     codes[i] = kNoHit;
     return;
   }
+  StackSpill<LDS_LEVELS> spill;
   SplitStack<LDS_LEVELS, 256> stack;
+  stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
   float4 o = org[i];
   const float4 d = dir[i];
@@ -174,7 +176,9 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
   PixelCounters cnt;
   if (x < W && y < H) {
     const int64_t i = int64_t(y) * W + x;
+    StackSpill<LDS_LEVELS> spill;
     SplitStack<LDS_LEVELS, BLOCK> stack;
+    stack.spill = spill.v;
     stack.lds = lds + threadIdx.x;
     stack.base = lds_base;
     uint32_t color;
@@ -213,7 +217,9 @@ __global__ __launch_bounds__(256) void RaytraceDepthKernel(uint32_t* __restrict_
   };
   const float bias = lod * dist(p1n, p2n);
   const float coef = (bias < 1e-10f) ? 0.0f : (lod * (dist(p1f, p2f) - dist(p1n, p2n)) / dist(p1n, p1f));
+  StackSpill<kStackLevels> spill;
   SplitStack<kStackLevels, 256> stack;
+  stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
   CastResult res;
   uint32_t fetches = 0;

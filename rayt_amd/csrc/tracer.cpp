@@ -82,7 +82,7 @@ int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
              "bad frame size");
   rt_tracer* t = new rt_tracer();
   t->cache = cache;
-  t->ctx = cache->ctx;
+  t->ctx = rayt::CtxRetain(cache->ctx);
   t->width = width;
   t->height = height;
   const uint64_t P = uint64_t(width) * height;
@@ -105,7 +105,9 @@ int rt_tracer_destroy(rt_tracer* t) {
   rt_buf_destroy(t->states);
   rt_feedback_destroy(t->feedback);
   for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+  rt_ctx* ctx = t->ctx;
   delete t;
+  rayt::CtxRelease(ctx);
   return RT_OK;
 }
 

@@ -178,3 +178,55 @@ def test_batch_views_over_several_scenes(gpu_ctx, scenes):
         np.testing.assert_array_equal(img[off:off + w * h], pi)
         np.testing.assert_array_equal(cod[off:off + w * h], pc)
     b.close()
+
+
+def test_objects_outlive_their_context_handle(gpu_ctx, scenes):
+    """Closing a context first leaves its cache, batch and buffers working (each holds a context reference)."""
+    ctx2 = api.Context(0)
+    tree = scenes[0][0]
+    m = api.GPUOctreeCacheManager(tree.info().blocks_count, tree, ctx2)
+    m.InitialFillCache()
+    b = api.BatchRenderer(m, 4)
+    b.config(tile_shape=0, lds_levels=8, wgs_per_cu=8, tree_depth=8)
+    vpi = camera_vpi(*POSES[0], aspect=160 / 120)
+    views, total, _ = b.make_views([(vpi, 160, 120)], flags=SHADOWS)
+    out = api.Buffer(ctx2, 4 * total)
+    ctx2.close()
+    b.render(views, 1, out)
+    img = out.read_all(np.uint32)
+    b.close()
+    m.close()
+    out.close()
+    pi, _ = _plain(gpu_ctx, scenes, (vpi, 160, 120, 0))
+    np.testing.assert_array_equal(img, pi)
+
+
+def test_server_pinned_delivery_and_release(gpu_ctx, scenes):
+    """Pinned results stay valid until released, hold their arena range, and equal the plain render."""
+    reqs = [(camera_vpi(*POSES[i], aspect=4 / 3), 640, 480, i % 2) for i in range(4)]
+    srv = _server(gpu_ctx, scenes, max_views_per_round=8, rounds_in_flight=2, framebuffer_bytes=8 * 640 * 480 * 2)
+    try:
+        ids = srv.submit_many(reqs, flags=SHADOWS, want_codes=True, pinned=True)
+        got = []
+        while len(got) < 2:
+            got += srv.poll(cap=8, timeout_ms=5000, keep_pinned=True)
+        assert len(got) == 2
+        # the arena is full of held results: nothing else can be admitted
+        assert srv.poll(cap=8, timeout_ms=300, keep_pinned=True) == []
+        for c, img, cod in got:
+            k = ids.index(c.id)
+            pi, pc = _plain(gpu_ctx, scenes, reqs[k])
+            np.testing.assert_array_equal(img, pi)
+            np.testing.assert_array_equal(cod, pc)
+            srv.release(c.id)
+        with pytest.raises(api.RaytError):
+            srv.release(got[0][0].id)  # already released
+        srv.drain(10000)
+        rest = srv.poll(cap=8)  # copied out and released by the wrapper
+        assert sorted(c.id for c, _, _ in rest) == sorted(set(ids) - {c.id for c, _, _ in got})
+        for c, img, cod in rest:
+            pi, pc = _plain(gpu_ctx, scenes, reqs[ids.index(c.id)])
+            np.testing.assert_array_equal(img, pi)
+            np.testing.assert_array_equal(cod, pc)
+    finally:
+        srv.close()
