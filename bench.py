@@ -29,6 +29,21 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_plain_depth11_1080p_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the plain kernel on this workload, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by the copy-kernel calibration recorded in the same file), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 MIX_SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
 MIX_WEIGHTS = [0.3, 0.3, 0.2, 0.15, 0.05]
 
@@ -50,8 +65,9 @@ def view_mix(n: int, seed: int):
     return specs
 
 
-def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: int = 7):
-    """64-view mix in one persistent launch vs the plain kernel launched once per view."""
+def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: int = 7, shape=None):
+    """64-view mix in one persistent launch vs the plain kernel launched once per view.
+    shape: (tile, LDS levels, workgroups per CU) instead of the autotuner's choice."""
     import numpy as np
     from rayt_amd import api
     flags = api.L.RT_FLAG_SHADOWS
@@ -64,8 +80,14 @@ def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: 
     st = b.render(views, nviews, out, codes, count=True)
     rays = st[0] + st[1]
     t = time.time()
-    best, best_ms = b.autotune(views, nviews, out, codes, repeats=2)
-    log(f"batched: {nviews} views, {total} pixels, {rays} rays; autotune {time.time() - t:.1f}s -> "
+    if shape:
+        best = shape
+        b.config(tile_shape=best[0], lds_levels=best[1], wgs_per_cu=best[2], tree_depth=depth)
+        b.render(views, nviews, out, codes)
+        best_ms = b.last_ms()
+    else:
+        best, best_ms = b.autotune(views, nviews, out, codes, repeats=2)
+    log(f"batched: {nviews} views, {total} pixels, {rays} rays; launch shape chosen in {time.time() - t:.1f}s -> "
         f"tile {api.BatchRenderer.TILE_SHAPES[best[0]]}, {best[1]} LDS levels, {best[2]} waves/CU ({best_ms:.2f} ms)")
     ms = []
     for _ in range(steps):
@@ -212,6 +234,139 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
             "requests_checked_bit_exact": check, "open_loop": open_loop, "burst": closed}
 
 
+def spline_poses(n: int, seed: int):
+    """n camera poses along a closed Catmull-Rom spline through 8 seeded control points above the terrain,
+    each looking at a point of the terrain near the centre that moves with the path (camera convention of
+    camera.cpp: forward = (sin yaw cos pitch, sin pitch, -cos yaw cos pitch))."""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ctrl = np.column_stack([rng.uniform(0.1, 0.9, 8), rng.uniform(0.55, 0.85, 8), rng.uniform(0.1, 0.9, 8)])
+
+    def point(s):
+        k = int(math.floor(s)) % 8
+        u = s - math.floor(s)
+        p0, p1, p2, p3 = (ctrl[(k + j - 1) % 8] for j in range(4))
+        return 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u * u
+                      + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
+
+    poses = []
+    for i in range(n):
+        s = 8.0 * i / n
+        p, q = point(s), point(s + 0.02)
+        target = np.array([0.5, 0.25, 0.5]) + 2.0 * (q - p)
+        v = target - p
+        yaw = math.degrees(math.atan2(v[0], -v[2]))
+        pitch = math.degrees(math.atan2(v[1], math.hypot(v[0], v[2])))
+        poses.append((yaw, pitch, tuple(float(x) for x in p)))
+    return poses
+
+
+def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, check_frames: int = 2):
+    """Fly-through over a block pool that holds `frac` of the tree's blocks: every frame runs passes, feedback
+    and cache transactions (faulted blocks uploaded from host memory under LRU eviction) until no ray faults.
+    The first `check_frames` frames are compared with the oracle (images and per-pass fault lists)."""
+    import numpy as np
+    from rayt_amd import api
+    info = tree.info()
+    max_blocks = max(8, int(info.blocks_count * frac))
+    cache = api.GPUOctreeCacheManager(max_blocks, tree, ctx)
+    cache.InitialFillCache()
+    tr = api.GPURayTracer(cache, W, H)
+    tr.set_frame_time_limit(1e9)
+    tr.set_flags(api.L.RT_FLAG_SHADOWS)
+    tr.set_seed(3)
+    poses = spline_poses(frames, seed=11)
+    oc = None
+    if check_frames:
+        import oracle as orc
+        import tempfile
+        path = os.path.join(tempfile.gettempdir(), f"rayt_bench_ooc_d{depth}.tree")
+        tree.write(path)
+        ot = orc.Tree(path)
+        oc = orc.Cache(ot, max_blocks)
+        oc.initial_fill()
+        rng, ef = 3, 1 << 16
+    passes = uploads = rays = 0
+    mism = 0
+    ctx.WaitForAll()
+    t0 = time.perf_counter()
+    t_check = 0.0
+    up0 = cache.counters()[0]
+    for k, (yaw, pitch, pos) in enumerate(poses):
+        vpi = api.Camera(yaw=yaw, pitch=pitch, position=pos, fov=60.0, aspect=W / H).view_proj_inv()
+        tr.RenderFrame(vpi)
+        stt = tr.frame_stats()
+        passes += stt[0]
+        rays += stt[4]
+        if oc is not None and k < check_frames:
+            a = time.perf_counter()
+            r = oc.render_frame(W, H, vpi, flags=api.L.RT_FLAG_SHADOWS, seed=rng, expected_faults=ef)
+            rng, ef = r["rng"], r["expected_faults"]
+            if tr.pass_faults() != r["pass_faults"] or not np.array_equal(tr.out_image(), r["image"]):
+                mism += 1
+            t_check += time.perf_counter() - a
+    ctx.WaitForAll()
+    wall = time.perf_counter() - t0 - t_check
+    uploads = cache.counters()[0] - up0
+    res = {"depth": depth, "blocks": info.blocks_count, "pool_blocks": max_blocks,
+           "pool_bytes": max_blocks * (info.bytes_in_block_content + 36), "frames": frames,
+           "width": W, "height": H, "frames_per_s": round(frames / wall, 2), "mrays_s": round(rays / wall / 1e6, 1),
+           "passes_per_frame": round(passes / frames, 2), "uploads_per_frame": round(uploads / frames, 1),
+           "frames_checked_bit_exact": check_frames - mism if oc is not None else 0}
+    log(f"out-of-core fly-through: {res}")
+    if mism:
+        raise SystemExit(f"{mism} out-of-core frames differ from the oracle")
+    return res
+
+
+def measure_cpu_plain_config(ctx, threads: int):
+    """The reference's CPU-runnable case: depth-8 terrain + 8 spheres, 640x480, 60 degree FOV, fixed pose,
+    primary rays only, fully resident, no LOD. The oracle on all threads and on one, and the GPU kernel."""
+    import numpy as np
+    from rayt_amd import api
+    import oracle as orc
+    spec = api.TerrainSpec(depth=8, seed=1, octaves=4, spheres=8, sphere_rmin=0.05, sphere_rmax=0.2)
+    tree = api.StoredOctree.terrain(spec, 8192)
+    cache = api.GPUOctreeCacheManager(tree.info().blocks_count, tree, ctx)
+    cache.InitialFillCache()
+    W, H = 640, 480
+    vpi = api.Camera(yaw=35.0, pitch=-25.0, position=(0.05, 0.75, 1.25), fov=60.0, aspect=W / H).view_proj_inv()
+    out, codes = api.Buffer(ctx, 4 * W * H), api.Buffer(ctx, 4 * W * H)
+    for _ in range(3):
+        api.render_plain(ctx, cache, W, H, vpi, out=out, codes=codes)
+    tm = api.EventTimer(ctx)
+    tm.start()
+    for _ in range(20):
+        api.render_plain(ctx, cache, W, H, vpi, out=out, codes=codes)
+    gpu_ms = tm.stop() / 20
+    links = cache.channel(0).read_all(np.uint32)
+    far = cache.far_pointers().read_all(np.uint32)
+    normals = cache.channel_by_name("normals").read_all(np.uint32)
+    colors = cache.channel_by_name("colors").read_all(np.uint32)
+
+    def cpu(th, secs):
+        runs, tc, r = 0, 0.0, None
+        while tc < secs or runs == 0:
+            a = time.perf_counter()
+            r = orc.render_plain(links, far, cache.root_node_index(), normals, colors, cache.nodes_in_block(), W, H,
+                                 vpi, threads=th)
+            tc += time.perf_counter() - a
+            runs += 1
+        return W * H * runs / tc / 1e6, r
+
+    cpu_all, r = cpu(threads, 3.0)
+    cpu_one, _ = cpu(1, 2.0)
+    same = bool(np.array_equal(out.read_all(np.uint32).reshape(H, W), r["image"]))
+    res = {"workload": "depth-8 terrain + 8 spheres, 640x480, primary rays, fixed pose", "gpu_mrays_s":
+           round(W * H / gpu_ms / 1e3, 1), "gpu_ms": round(gpu_ms, 4), "cpu_mrays_s": round(cpu_all, 2),
+           "cpu_threads": threads, "cpu_1thread_mrays_s": round(cpu_one, 2), "gpu_equals_oracle": same}
+    log(f"CPU-plain config: {res}")
+    if not same:
+        raise SystemExit("GPU image differs from the oracle on the CPU-plain config")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +386,15 @@ def main():
                     help="depths of the resident served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run a device-to-device copy of MB MiB (3 times) as the byte-counter reference")
+    ap.add_argument("--no-outofcore", action="store_true")
+    ap.add_argument("--ooc-depth", type=int, default=12)
+    ap.add_argument("--ooc-frac", type=float, default=0.25, help="pool size as a fraction of the tree's blocks")
+    ap.add_argument("--ooc-frames", type=int, default=128)
+    ap.add_argument("--no-cpu-plain", action="store_true")
+    ap.add_argument("--plain-shape", default="", metavar="WAVES,LEVELS",
+                    help="plain kernel launch shape instead of the sweep (for profiling one kernel variant)")
+    ap.add_argument("--batch-shape", default="", metavar="TILE,LEVELS,WGS",
+                    help="batched launch shape instead of the autotuner (tile 0=8x8, 1=16x4, 2=32x2)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -275,20 +439,23 @@ def main():
 
     # Launch shape of the plain kernel: the fastest of a small sweep (outside the timed region).
     pout, pcodes = api.Buffer(ctx, 4 * W * H), api.Buffer(ctx, 4 * W * H)
-    shapes = []
-    for waves in (1, 4):
-        for levels in (10, 12, 16):
-            api.plain_config(ctx, levels, waves, args.depth)
-            api.render_plain(ctx, cache, W, H, vpi, lod=lod, flags=flags, out=pout, codes=pcodes)
-            tm = api.EventTimer(ctx)
-            tm.start()
-            for _ in range(5):
+    if args.plain_shape:
+        plain_waves, plain_levels = (int(x) for x in args.plain_shape.split(","))
+    else:
+        shapes = []
+        for waves in (1, 4):
+            for levels in (10, 12, 16):
+                api.plain_config(ctx, levels, waves, args.depth)
                 api.render_plain(ctx, cache, W, H, vpi, lod=lod, flags=flags, out=pout, codes=pcodes)
-            shapes.append((tm.stop() / 5, waves, levels))
-    shapes.sort()
-    plain_ms, plain_waves, plain_levels = shapes[0]
+                tm = api.EventTimer(ctx)
+                tm.start()
+                for _ in range(5):
+                    api.render_plain(ctx, cache, W, H, vpi, lod=lod, flags=flags, out=pout, codes=pcodes)
+                shapes.append((tm.stop() / 5, waves, levels))
+        shapes.sort()
+        _, plain_waves, plain_levels = shapes[0]
+        log("plain launch shapes (ms, waves/WG, LDS levels): " + ", ".join(f"({m:.3f}, {w}, {l})" for m, w, l in shapes))
     api.plain_config(ctx, plain_levels, plain_waves, args.depth)
-    log("plain launch shapes (ms, waves/WG, LDS levels): " + ", ".join(f"({m:.3f}, {w}, {l})" for m, w, l in shapes))
 
     tr = api.GPURayTracer(cache, W, H)
     tr.set_frame_time_limit(1e9)
@@ -314,7 +481,9 @@ def main():
         f"({rays / avg_kernel_ms / 1e3:.0f} Mrays/s), {achieved_gbs:.0f} GB/s algorithmic")
 
     img = tr.out_image()
-    batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3))
+    batch_shape = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
+    batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3),
+                                                           shape=batch_shape)
     served = None
     if not args.no_served:
         scenes = []
@@ -328,6 +497,18 @@ def main():
             log(f"served scene depth {d} (seed {2 + k}): {scenes[-1][0].info().blocks_count} blocks, "
                 f"built in {time.time() - t:.1f}s")
         served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst)
+        del scenes
+    outofcore = None
+    if not args.no_outofcore:
+        t = time.time()
+        ospec = api.TerrainSpec(depth=args.ooc_depth, seed=14, octaves=6)
+        otree = api.StoredOctree.terrain(ospec, 8192)
+        log(f"out-of-core scene depth {args.ooc_depth} (6 octaves): {otree.info().blocks_count} blocks, "
+            f"built in {time.time() - t:.1f}s")
+        outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H)
+        del otree
+    cpu_plain = None if args.no_cpu_plain or args.no_cpu_baseline else \
+        measure_cpu_plain_config(ctx, max(1, min(16, os.cpu_count() or 1)))
     import oracle as orc  # CPU baseline + one-frame check of the measured output
 
     cpu = None
@@ -372,13 +553,15 @@ def main():
                    "primary_rays": primary, "shadow_rays": shadow,
                    "plain_launch": {"waves_per_workgroup": plain_waves, "lds_stack_levels": plain_levels}},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "kernel": "RenderPlainKernel", "kernel_ms": round(avg_kernel_ms, 4),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "kernel_mrays_s": round(rays / avg_kernel_ms / 1e3, 1),
         "batched": batched,
         "served": served,
+        "out_of_core": outofcore,
+        "cpu_plain_config": cpu_plain,
     }
     print(json.dumps(out), flush=True)
 

@@ -231,59 +231,88 @@ __device__ __forceinline__ int WaveMax(int v) {
   return v;
 }
 
-// Marks the distinct codes in the bitmaps. Within a wave only one lane per
-// distinct value touches the bitmap (a ballot loop over the values present),
-// and a bitmap word is read before it is or-ed, so the few hot words see
-// atomics only until their bits are set. The maxima / flags of each workgroup
-// go to its own slot of `partials` (no shared counter), reduced by the walk.
-__global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
-                                                       uint32_t* __restrict__ fault_bits, uint32_t* __restrict__ hit_bits,
-                                                       int4* __restrict__ partials) {
+// Marks the distinct codes in the bitmaps. Each workgroup takes a contiguous
+// chunk of kMarkItems codes per thread and inserts every code into an LDS
+// hash set of the workgroup (one compare-and-swap per lane and code: lanes
+// with equal codes collide on one LDS word, which the hardware serialises
+// inside the instruction). After the chunk, every distinct value of the
+// workgroup sets its bitmap bit with one atomic that returns nothing, so no
+// wave waits on global memory and a hot bitmap word sees one atomic per
+// workgroup. A value that finds no free slot within kMarkProbe probes sets its
+// bit directly. The maxima / flags of each workgroup go to its own slot of
+// `partials`, reduced by the walk.
+constexpr int kMarkBlock = 512;
+constexpr int kMarkItems = 16;
+constexpr int kMarkBatch = 8;  // codes per thread loaded together
+constexpr int kMarkSlots = 4096;
+constexpr int kMarkProbe = 32;
+constexpr uint32_t kMarkEmpty = 0xffffffffu;
+constexpr uint32_t kMarkFaultKey = 0x40000000u;  // key = hit code, or fault block | this bit
+
+__device__ __forceinline__ void MarkBit(uint32_t key, uint32_t* fault_bits, uint32_t* hit_bits) {
+  if (key & kMarkFaultKey) {
+    const uint32_t f = key & ~kMarkFaultKey;
+    atomicOr(&fault_bits[f >> 5], 1u << (f & 31u));
+  } else {
+    atomicOr(&hit_bits[key >> 5], 1u << (key & 31u));
+  }
+}
+
+__global__ __launch_bounds__(kMarkBlock) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
+                                                              uint32_t* __restrict__ fault_bits,
+                                                              uint32_t* __restrict__ hit_bits,
+                                                              int4* __restrict__ partials) {
+  __shared__ uint32_t table[kMarkSlots];
   __shared__ int red[4];  // max fault, max hit, NO_HIT seen, overflow
+  for (int j = threadIdx.x; j < kMarkSlots; j += kMarkBlock) table[j] = kMarkEmpty;
   if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   int mf = -1, mh = -1;
   bool nohit = false, over = false;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
-    const bool valid = i < n;
-    int code = valid ? codes[i] : kNoHit;
-    if (block_size > 0 && code > 0 && code != kNoHit) code = ((((code >> 1) - 1) / block_size + 1) << 1) | (code & 1);
-    bool mark = false;
-    if (valid) {
-      if (code == kNoHit) {
-        nohit = true;
-      } else if (code < 0) {
-        const uint32_t f = uint32_t(-(code + 1));
-        if (f >= kFaultDomain) {
-          over = true;
-        } else {
-          mark = true;
-          mf = max(mf, int(f));
-        }
-      } else if (uint32_t(code) >= kHitDomain) {
-        over = true;
-      } else {
-        mark = true;
-        mh = max(mh, code);
+  const int chunk = kMarkBlock * kMarkItems;
+  for (int base = blockIdx.x * chunk; base < n; base += gridDim.x * chunk) {
+    for (int it0 = 0; it0 < kMarkItems; it0 += kMarkBatch) {
+      int loaded[kMarkBatch];  // the batch's loads are in flight before the first is used
+#pragma unroll
+      for (int k = 0; k < kMarkBatch; ++k) {
+        const int i = base + (it0 + k) * kMarkBlock + threadIdx.x;  // coalesced: consecutive lanes, consecutive codes
+        loaded[k] = i < n ? codes[i] : kNoHit;
       }
-    }
-    uint64_t pending = __ballot(mark);
-    while (pending) {
-      const int leader = __ffsll((unsigned long long)pending) - 1;
-      const int v = __shfl(code, leader, 64);
-      pending &= ~__ballot(mark && code == v);
-      if (lane == leader) {
-        if (v < 0) {
-          const uint32_t f = uint32_t(-(v + 1));
-          const uint32_t bit = 1u << (f & 31u);
-          uint32_t* w = &fault_bits[f >> 5];
-          if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
-        } else {
-          const uint32_t bit = 1u << (uint32_t(v) & 31u);
-          uint32_t* w = &hit_bits[uint32_t(v) >> 5];
-          if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
+#pragma unroll
+      for (int k = 0; k < kMarkBatch; ++k) {
+        const bool valid = base + (it0 + k) * kMarkBlock + int(threadIdx.x) < n;
+        int code = loaded[k];
+        if (block_size > 0 && code > 0 && code != kNoHit)
+          code = ((((code >> 1) - 1) / block_size + 1) << 1) | (code & 1);
+        uint32_t key = kMarkEmpty;
+        if (valid) {
+          if (code == kNoHit) {
+            nohit = true;
+          } else if (code < 0) {
+            const uint32_t f = uint32_t(-(code + 1));
+            if (f >= kFaultDomain) {
+              over = true;
+            } else {
+              key = f | kMarkFaultKey;
+              mf = max(mf, int(f));
+            }
+          } else if (uint32_t(code) >= kHitDomain) {
+            over = true;
+          } else {
+            key = uint32_t(code);
+            mh = max(mh, code);
+          }
+        }
+        if (key != kMarkEmpty) {
+          uint32_t h = (key * 0x9E3779B1u) >> 20;  // 12-bit slot
+          int p = 0;
+          for (; p < kMarkProbe; ++p) {
+            const uint32_t old = atomicCAS(&table[h], kMarkEmpty, key);
+            if (old == kMarkEmpty || old == key) break;
+            h = (h + 1) & (kMarkSlots - 1);
+          }
+          if (p == kMarkProbe) MarkBit(key, fault_bits, hit_bits);
         }
       }
     }
@@ -298,6 +327,10 @@ __global__ __launch_bounds__(256) void MarkCodesKernel(const int* __restrict__ c
     if (any_over) red[3] = 1;
   }
   __syncthreads();
+  for (int j = threadIdx.x; j < kMarkSlots; j += kMarkBlock) {
+    const uint32_t v = table[j];
+    if (v != kMarkEmpty) MarkBit(v, fault_bits, hit_bits);
+  }
   if (threadIdx.x == 0) partials[blockIdx.x] = make_int4(red[0], red[1], red[2], red[3]);
 }
 
@@ -499,8 +532,8 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   }
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
-  const int grid = std::min(Grid(count_, 256), max_mark_blocks_);
-  MarkCodesKernel<<<dim3(grid), dim3(256), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_,
+  const int grid = std::min(Grid(count_, kMarkBlock * kMarkItems), max_mark_blocks_);
+  MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_,
                                                    static_cast<int4*>(partials_));
   WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
                                                    static_cast<const int4*>(partials_), grid);
