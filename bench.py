@@ -262,10 +262,12 @@ def spline_poses(n: int, seed: int):
     return poses
 
 
-def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, check_frames: int = 2):
+def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, check_frames: int = 2,
+                        lod: float = 1.0):
     """Fly-through over a block pool that holds `frac` of the tree's blocks: every frame runs passes, feedback
-    and cache transactions (faulted blocks uploaded from host memory under LRU eviction) until no ray faults.
-    The first `check_frames` frames are compared with the oracle (images and per-pass fault lists)."""
+    and cache transactions (faulted blocks uploaded from host memory under LRU eviction) until no ray faults
+    or the session fills the pool. LOD voxel size 1 (the application's default). The first `check_frames`
+    frames are compared with the oracle (images and per-pass fault lists)."""
     import numpy as np
     from rayt_amd import api
     info = tree.info()
@@ -274,6 +276,7 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     cache.InitialFillCache()
     tr = api.GPURayTracer(cache, W, H)
     tr.set_frame_time_limit(1e9)
+    tr.set_lod_voxel_size(lod)
     tr.set_flags(api.L.RT_FLAG_SHADOWS)
     tr.set_seed(3)
     poses = spline_poses(frames, seed=11)
@@ -301,7 +304,7 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
         rays += stt[4]
         if oc is not None and k < check_frames:
             a = time.perf_counter()
-            r = oc.render_frame(W, H, vpi, flags=api.L.RT_FLAG_SHADOWS, seed=rng, expected_faults=ef)
+            r = oc.render_frame(W, H, vpi, lod=lod, flags=api.L.RT_FLAG_SHADOWS, seed=rng, expected_faults=ef)
             rng, ef = r["rng"], r["expected_faults"]
             if tr.pass_faults() != r["pass_faults"] or not np.array_equal(tr.out_image(), r["image"]):
                 mism += 1
@@ -309,11 +312,15 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     ctx.WaitForAll()
     wall = time.perf_counter() - t0 - t_check
     uploads = cache.counters()[0] - up0
-    res = {"depth": depth, "blocks": info.blocks_count, "pool_blocks": max_blocks,
+    checked = check_frames - mism if oc is not None else 0
+    if oc is not None:
+        del oc, ot
+        os.remove(path)
+    res = {"depth": depth, "blocks": info.blocks_count, "pool_blocks": max_blocks, "lod_voxel_size": lod,
            "pool_bytes": max_blocks * (info.bytes_in_block_content + 36), "frames": frames,
            "width": W, "height": H, "frames_per_s": round(frames / wall, 2), "mrays_s": round(rays / wall / 1e6, 1),
            "passes_per_frame": round(passes / frames, 2), "uploads_per_frame": round(uploads / frames, 1),
-           "frames_checked_bit_exact": check_frames - mism if oc is not None else 0}
+           "frames_checked_bit_exact": checked}
     log(f"out-of-core fly-through: {res}")
     if mism:
         raise SystemExit(f"{mism} out-of-core frames differ from the oracle")
@@ -391,8 +398,9 @@ def main():
     ap.add_argument("--ooc-frac", type=float, default=0.25, help="pool size as a fraction of the tree's blocks")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
-    ap.add_argument("--plain-shape", default="", metavar="WAVES,LEVELS",
-                    help="plain kernel launch shape instead of the sweep (for profiling one kernel variant)")
+    ap.add_argument("--plain-shape", default="4,10", metavar="WAVES,LEVELS|sweep",
+                    help="plain kernel launch shape (the one the committed HBM-traffic profile measured), or "
+                         "'sweep' to time 1/4 waves x 10/12/16 LDS levels and keep the fastest")
     ap.add_argument("--batch-shape", default="", metavar="TILE,LEVELS,WGS",
                     help="batched launch shape instead of the autotuner (tile 0=8x8, 1=16x4, 2=32x2)")
     args = ap.parse_args()
@@ -439,7 +447,7 @@ def main():
 
     # Launch shape of the plain kernel: the fastest of a small sweep (outside the timed region).
     pout, pcodes = api.Buffer(ctx, 4 * W * H), api.Buffer(ctx, 4 * W * H)
-    if args.plain_shape:
+    if args.plain_shape != "sweep":
         plain_waves, plain_levels = (int(x) for x in args.plain_shape.split(","))
     else:
         shapes = []
