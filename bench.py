@@ -398,6 +398,7 @@ def main():
     ap.add_argument("--ooc-frac", type=float, default=0.25, help="pool size as a fraction of the tree's blocks")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
+    ap.add_argument("--dump", default="", metavar="DIR", help="write the measured frame and its feedback codes as PNG")
     ap.add_argument("--plain-shape", default="4,10", metavar="WAVES,LEVELS|sweep",
                     help="plain kernel launch shape (the one the committed HBM-traffic profile measured), or "
                          "'sweep' to time 1/4 waves x 10/12/16 LDS levels and keep the fastest")
@@ -489,6 +490,11 @@ def main():
         f"({rays / avg_kernel_ms / 1e3:.0f} Mrays/s), {achieved_gbs:.0f} GB/s algorithmic")
 
     img = tr.out_image()
+    if args.dump:
+        from rayt_amd import imageio
+        os.makedirs(args.dump, exist_ok=True)
+        imageio.write_png(os.path.join(args.dump, "plain_frame.png"), img)
+        imageio.write_png(os.path.join(args.dump, "plain_codes.png"), imageio.code_colors(tr.codes()))
     batch_shape = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
     batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3),
                                                            shape=batch_shape)

@@ -41,3 +41,29 @@ def test_struct_layouts_match_c(tmp_path):
         assert got[(cname, "sizeof")] == C.sizeof(py), cname
         for field, _ in py._fields_:
             assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
+
+def test_image_dumps_round_trip(tmp_path):
+    """PNG / PPM dumps of out_image (the replacement of the GL display path) decode to the same pixels."""
+    import zlib
+    import numpy as np
+    from rayt_amd import imageio
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 2**32, (5, 7), dtype=np.uint64).astype(np.uint32)
+    p = tmp_path / "a.png"
+    imageio.write_png(str(p), img)
+    data = p.read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    ln = int.from_bytes(data[33:37], "big")
+    assert data[37:41] == b"IDAT"
+    raw = zlib.decompress(data[41:41 + ln])
+    rows = np.frombuffer(raw, np.uint8).reshape(5, 1 + 7 * 4)
+    assert (rows[:, 0] == 0).all()
+    np.testing.assert_array_equal(rows[:, 1:].reshape(5, 7, 4), imageio.rgba_bytes(img))
+    q = tmp_path / "a.ppm"
+    imageio.write_ppm(str(q), img)
+    body = q.read_bytes().split(b"\n", 3)[3]
+    np.testing.assert_array_equal(np.frombuffer(body, np.uint8).reshape(5, 7, 3), imageio.rgba_bytes(img)[:, :, :3])
+    codes = np.array([[-1, 0x7FFFFFFF, (3 << 1) | 1, 3 << 1]], np.int32)
+    cc = imageio.code_colors(codes)
+    assert cc[0, 0] == 0xFF0000FF and cc[0, 1] == 0xFF000000 and cc[0, 2] != cc[0, 3]
