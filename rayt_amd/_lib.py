@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "librayt_hip.so")
+# RAYT_HIP_LIB selects another build of the same library (for comparing builds side by side).
+LIB_PATH = os.environ.get("RAYT_HIP_LIB") or os.path.join(PKG_DIR, "librayt_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "rayt_hip.h")
 
 RT_OK = 0

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
             const PrimaryRay r = InitRay(v.m, x, y, v.width, v.height, v.lod);
             ox = r.origin.x, oy = r.origin.y, oz = r.origin.z;
             dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
-            TravInit(tr, root, ox, oy, oz, dx, dy, dz, r.direction.w, r.origin.w);
+            TravInit(tr, pool, root, ox, oy, oz, dx, dy, dz, r.direction.w, r.origin.w);
             pix = v.pixel_offset + uint64_t(y) * v.width + x;
             flags = v.flags;
             phase = 0;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
           // same origin rule as ShadowOccluded
           const float eps = __uint_as_float(uint32_t(127 - res.node_depth - 2) << 23);
           const float tb = cl_max(0.0f, res.hit_t - eps);
-          TravInit(tr, root, ox + dx * tb, oy + dy * tb, oz + dz * tb, lt.shadow_dir[0], lt.shadow_dir[1],
+          TravInit(tr, pool, root, ox + dx * tb, oy + dy * tb, oz + dz * tb, lt.shadow_dir[0], lt.shadow_dir[1],
                    lt.shadow_dir[2], 0.0f, 0.0f);
           phase = 1;
           if (COUNT) ++cnt.shadow_rays;

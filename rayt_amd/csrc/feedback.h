@@ -51,6 +51,10 @@ class FeedbackExtractor {
   uint32_t* hit_bits_ = nullptr;
   void* state_ = nullptr;
   void* host_state_ = nullptr;
+  // Pinned head of the unique list, read with the state (kPrefixInts codes).
+  static constexpr int kPrefixInts = 16384;
+  void* host_prefix_ = nullptr;
+  int prefix_len_ = 0;
   int* unique_ = nullptr;
   int* temp_ = nullptr;
   int* index_ = nullptr;

@@ -474,6 +474,8 @@ FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(CtxRetain(ct
   (void)hipMemcpyAsync(state_, &init, sizeof(init), hipMemcpyHostToDevice, ctx_->stream);
   hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_state_), sizeof(MarkState), hipHostMallocDefault);
   if (e != hipSuccess) throw std::runtime_error("hipHostMalloc");
+  e = hipHostMalloc(&host_prefix_, size_t(kPrefixInts) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) throw std::runtime_error("hipHostMalloc");
   host_sorted_.resize(size_t(std::max(count, 1)) + 1);
   (void)hipStreamSynchronize(ctx_->stream);
 }
@@ -488,6 +490,7 @@ FeedbackExtractor::~FeedbackExtractor() {
   (void)hipFree(index_);
   (void)hipFree(partials_);
   (void)hipHostFree(host_state_);
+  (void)hipHostFree(host_prefix_);
   CtxRelease(ctx_);
 }
 
@@ -538,10 +541,15 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
                                                    static_cast<const int4*>(partials_), grid);
   RT_HIP_CHECK(hipGetLastError());
+  // One round trip: the state and, speculatively, the head of the unique list
+  // (usually all of it) come back together into pinned memory.
   RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));
+  prefix_len_ = std::min(count_ + 1, kPrefixInts);
+  RT_HIP_CHECK(hipMemcpyAsync(host_prefix_, unique_, size_t(prefix_len_) * 4, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipStreamSynchronize(s));
   const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
   if (hs.overflow) {
+    prefix_len_ = 0;  // the sort path rewrites the list
     if (!block_level) NodeToBlockKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, nodes_in_block, codes);
     int rc = UniqueBySort(codes);
     if (rc) return rc;
@@ -573,7 +581,6 @@ int FeedbackExtractor::ReadLists() {
     window_start = int(NextRandom() % uint32_t(hits_length));
     window_len = std::min(hits_to_extract_, hits_length);
   }
-  if (size > 0) RT_HIP_CHECK(hipMemcpyAsync(host_sorted_.data(), unique_, size_t(size) * 4, hipMemcpyDeviceToHost, s));
   std::vector<std::pair<int, int>> segs;
   if (window_len > 0) {
     if (window_start + window_len <= hits_length) {
@@ -584,13 +591,27 @@ int FeedbackExtractor::ReadLists() {
     }
   }
   window_.resize(size_t(window_len));
-  int w = 0;
-  for (auto& sg : segs) {
-    RT_HIP_CHECK(hipMemcpyAsync(window_.data() + w, unique_ + size + sg.first, size_t(sg.second) * 4,
-                                hipMemcpyDeviceToHost, s));
-    w += sg.second;
+  // Everything the reads below need lies in the prefix fetched with the state
+  // when the list is short; otherwise read it as the reference does.
+  if (size + hits_length <= prefix_len_) {
+    const int* pre = static_cast<const int*>(host_prefix_);
+    std::copy(pre, pre + size, host_sorted_.begin());
+    int w = 0;
+    for (auto& sg : segs) {
+      std::copy(pre + size + sg.first, pre + size + sg.first + sg.second, window_.begin() + w);
+      w += sg.second;
+    }
+  } else {
+    if (size > 0)
+      RT_HIP_CHECK(hipMemcpyAsync(host_sorted_.data(), unique_, size_t(size) * 4, hipMemcpyDeviceToHost, s));
+    int w = 0;
+    for (auto& sg : segs) {
+      RT_HIP_CHECK(hipMemcpyAsync(window_.data() + w, unique_ + size + sg.first, size_t(sg.second) * 4,
+                                  hipMemcpyDeviceToHost, s));
+      w += sg.second;
+    }
+    RT_HIP_CHECK(hipStreamSynchronize(s));
   }
-  RT_HIP_CHECK(hipStreamSynchronize(s));
   for (int i = 0; i < size; ++i) {
     const int v = host_sorted_[i];
     if (v < 0)

@@ -111,6 +111,15 @@ struct PoolView {
 // CastRay runs the steps to the end; the batched kernel interleaves steps of
 // different rays in one lane. `fetches` counts executed node-link and
 // far-pointer loads (the F of the roofline accounting).
+//
+// The reference reloads the parent's descriptor at the top of an iteration
+// whenever it is zero, which happens at the start and after every pop
+// (raycast.cl:115). Here the load is issued where the zero would be set (at
+// the start and at the end of a pop), so its latency overlaps the rest of the
+// step; the loaded word is the same. For the fetch count, `pending_fetch`
+// carries the load to the next iteration's top, after the iteration-limit
+// test, where the reference performs it; a descriptor that is still zero is
+// counted again at every iteration, as the reference reloads it.
 struct Trav {
   float tx_coef, ty_coef, tz_coef, tx_bias, ty_bias, tz_bias;
   float ray_size_coef, ray_size_bias;
@@ -118,10 +127,12 @@ struct Trav {
   float posx, posy, posz, scale_exp2;
   uint32_t parent, child_descriptor, idx, octant_mask;
   int scale, iter;
+  uint32_t pending_fetch;
 };
 
-__device__ __forceinline__ void TravInit(Trav& s, uint32_t root_node, float px, float py, float pz, float dx, float dy,
-                                         float dz, float ray_size_coef, float ray_size_bias) {
+__device__ __forceinline__ void TravInit(Trav& s, const PoolView& pool, uint32_t root_node, float px, float py,
+                                         float pz, float dx, float dy, float dz, float ray_size_coef,
+                                         float ray_size_bias) {
   px += 1.0f;
   py += 1.0f;
   pz += 1.0f;
@@ -143,7 +154,8 @@ __device__ __forceinline__ void TravInit(Trav& s, uint32_t root_node, float px, 
   s.h = s.t_max;
   s.t_min = cl_fmax(s.t_min, 0.0f);
   s.parent = root_node;
-  s.child_descriptor = 0;
+  s.child_descriptor = pool.links[root_node];
+  s.pending_fetch = 1;
   s.idx = 0;
   s.posx = s.posy = s.posz = 1.0f;
   s.scale = kSMax - 1;
@@ -176,15 +188,19 @@ template <class Stack, bool COUNT>
 __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& stack, uint32_t& fetches) {
   if (s.scale >= kSMax) return kMiss;
   if (++s.iter > kIterLimitSteps) return kIterLimit;
-  if (s.child_descriptor == 0) {
-    s.child_descriptor = pool.links[s.parent];
-    if (COUNT) ++fetches;
+  if (COUNT) {
+    fetches += (s.pending_fetch || s.child_descriptor == 0) ? 1u : 0u;
+    s.pending_fetch = 0;
   }
   const uint32_t desc = s.child_descriptor;
   const float tx_corner = s.posx * s.tx_coef - s.tx_bias;
   const float ty_corner = s.posy * s.ty_coef - s.ty_bias;
   const float tz_corner = s.posz * s.tz_coef - s.tz_bias;
-  const float tc_max = cl_fmin(cl_fmin(tx_corner, ty_corner), tz_corner);
+  // fmin of raycast.cl as one v_min_f32 (IEEE minNum). minNum returns the
+  // other operand when one is NaN, as OpenCL fmin does; it may differ from
+  // fmin only on (+0, -0), and a corner is never -0: posx * coef is nonzero
+  // (posx >= 1, |coef| >= 1) and x - y rounds to +0 when x == y.
+  const float tc_max = __builtin_fminf(__builtin_fminf(tx_corner, ty_corner), tz_corner);
 
   const uint32_t child_masks = 1u << (s.idx ^ s.octant_mask);
   const bool want = (desc & child_masks) != 0 && s.t_min <= s.t_max;
@@ -259,7 +275,8 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
         s.posz = __uint_as_float(shz << scale);
         s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
         s.h = 0.0f;
-        s.child_descriptor = 0;
+        s.child_descriptor = pool.links[s.parent];
+        s.pending_fetch = 1;
       }
     }
   }
@@ -281,7 +298,7 @@ __device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node
                                         float dx, float dy, float dz, float ray_size_coef, float ray_size_bias,
                                         Stack& stack, CastResult& res, uint32_t& fetches) {
   Trav s;
-  TravInit(s, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
+  TravInit(s, pool, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
   int status;
   do {
     status = TravStep<Stack, COUNT>(s, pool, stack, fetches);
