@@ -271,6 +271,8 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     import numpy as np
     from rayt_amd import api
     info = tree.info()
+    # The fused single-pass frames run the plain kernel: declare this tree's depth to it.
+    api.plain_config(ctx, 16, 4, depth)
     max_blocks = max(8, int(info.blocks_count * frac))
     cache = api.GPUOctreeCacheManager(max_blocks, tree, ctx)
     cache.InitialFillCache()

@@ -132,7 +132,10 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
 /* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context:
  * lds_stack_levels (8, 10, 12, 14, 16, 22) traversal-stack levels held in LDS, wg_waves 1 (one wave per
  * workgroup, 8x8 pixels) or 4 (16x16 pixels), tree_depth of the scene (0 = unknown) to keep the deepest
- * levels in LDS. Default: 16 levels, 4 waves. Results do not depend on the shape. */
+ * levels in LDS. When tree_depth - 1 <= lds_stack_levels the whole stack lives in LDS (no private spill
+ * array). The declared depth must not be smaller than the tree's: a primary ray that would go deeper ends
+ * with the iteration-limit error (red pixel, HIT_CODE_NO_HIT code), a shadow ray as unoccluded; no access
+ * leaves the stack. Default: 16 levels, 4 waves, depth unknown. Results do not depend on the shape. */
 int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int wg_waves, int tree_depth);
 
 /* ------------------------------------------------------ GPU algorithms
@@ -285,8 +288,10 @@ int rt_batch_destroy(rt_batch* batch);
 /* Registers another resident scene of the same context; returns its scene index (>= 1) or a negative error. */
 int rt_batch_add_scene(rt_batch* batch, rt_cache* cache);
 /* Launch shape: tile_shape 0 = 8x8, 1 = 16x4, 2 = 32x2 pixels per wave; lds_levels (8, 10, 12, 16) deepest
- * traversal-stack levels kept in LDS; wgs_per_cu one-wave workgroups per compute unit (1..32); tree_depth of the
- * scene (places the LDS window on the deepest levels). */
+ * traversal-stack levels kept in LDS; wgs_per_cu one-wave workgroups per compute unit (1..32); tree_depth of
+ * the batch's scenes (0..22; 0 = unknown: the stack spills to a private array beyond the LDS window). With
+ * tree_depth - 1 <= lds_levels the whole stack lives in LDS, under the same rule on the declared depth as
+ * rt_plain_config. */
 int rt_batch_config(rt_batch* batch, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth);
 /* Enqueues one batched launch on the cache's context stream. stats (may be NULL) makes it the counting variant
  * and receives {primary rays, shadow rays, node fetches, hits}. */

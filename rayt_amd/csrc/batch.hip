@@ -56,7 +56,7 @@ struct Vec4B {
   float v[4];
 };
 
-template <int TW, int LDS_LEVELS, bool COUNT>
+template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL>
 __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ views, int nviews,
                                                   const DevScene* __restrict__ scenes,
                                                   const uint32_t* __restrict__ tile_begin, uint32_t total_tiles,
@@ -65,8 +65,8 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
                                                   unsigned long long* __restrict__ stats) {
   constexpr int TH = 64 / TW;
   __shared__ uint2 lds[LDS_LEVELS * 64];
-  StackSpill<LDS_LEVELS> spill;
-  SplitStack<LDS_LEVELS, 64> stack;
+  StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
+  SplitStack<LDS_LEVELS, 64, SPILL> stack;
   stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
       continue;  // every claimed pixel so far was outside its view; claim more
     }
     if (!active) continue;
-    const int status = TravStep<SplitStack<LDS_LEVELS, 64>, COUNT>(tr, pool, stack, fetches);
+    const int status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT>(tr, pool, stack, fetches);
     if (status == kRunning) continue;
     CastResult res;
     TravResult(tr, status, res);
@@ -239,17 +239,21 @@ struct LaunchArgs {
   unsigned long long* stats;
 };
 
-template <int TW, int L, bool C>
+template <int TW, int L, bool C, bool SP>
 void Launch(const LaunchArgs& a) {
-  BatchKernel<TW, L, C><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles,
+  BatchKernel<TW, L, C, SP><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles,
                                                            a.counters, a.out, a.codes, a.bg, a.lt, a.stats);
 }
 
+// spill = false selects the window-only stack (every scene fits the LDS window).
 template <bool C>
-bool Dispatch(int tw, int levels, const LaunchArgs& a) {
+bool Dispatch(int tw, int levels, bool spill, const LaunchArgs& a) {
 #define RT_B(TW, L)                   \
   if (tw == TW && levels == L) {      \
-    Launch<TW, L, C>(a);              \
+    if (spill)                        \
+      Launch<TW, L, C, true>(a);      \
+    else                              \
+      Launch<TW, L, C, false>(a);     \
     return true;                      \
   }
   RT_B(8, 8) RT_B(8, 10) RT_B(8, 12) RT_B(8, 16)
@@ -291,7 +295,7 @@ struct rt_batch {
   int tile_shape = 0;   // 0: 8x8, 1: 16x4, 2: 32x2
   int lds_levels = 12;  // 8, 10, 12 or 16
   int wgs_per_cu = 16;  // one-wave workgroups per compute unit
-  int tree_depth = 11;  // depth assumed for scenes registered without one
+  int tree_depth = 0;   // depth assumed for scenes registered without one (0: unknown)
   std::vector<rt_cache*> scenes;
   std::vector<int> depths;  // per scene; 0 = tree_depth
   DevScene* d_scenes = nullptr;
@@ -379,11 +383,15 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
     RT_REQUIRE(tiles < (1ull << 31), RT_ERR_RANGE, "too many tiles in one batch");
   }
   b->h_tile_begin[nviews] = uint32_t(tiles);
+  // The window-only stack needs every scene's declared depth to fit the LDS window.
+  bool spill = false;
   for (size_t i = 0; i < b->scenes.size(); ++i) {
     rt_cache* c = b->scenes[i];
     RT_REQUIRE(c->ctx == b->ctx, RT_ERR_INVALID, "scene belongs to another context");
     RT_REQUIRE(c->mgr->root_node_index() >= 0, RT_ERR_STATE, "scene has no root block in its pool");
-    b->h_scenes[i] = SceneOf(c, b->depths[i] > 0 ? b->depths[i] : b->tree_depth, b->lds_levels);
+    const int depth = b->depths[i] > 0 ? b->depths[i] : b->tree_depth;
+    spill = spill || depth <= 0 || depth - 1 > b->lds_levels;
+    b->h_scenes[i] = SceneOf(c, depth, b->lds_levels);
   }
   hipStream_t s = b->stream;
   RT_HIP_CHECK(hipMemcpyAsync(b->d_views, b->h_views, sizeof(DevView) * nviews, hipMemcpyHostToDevice, s));
@@ -411,7 +419,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   a.grid = dim3(uint32_t((std::max<uint64_t>(want, 1) + 7) & ~uint64_t(7)));
   a.stats = count ? b->d_stats : nullptr;
   RT_HIP_CHECK(hipEventRecord(b->ev0, s));
-  const bool ok = count ? Dispatch<true>(tw, b->lds_levels, a) : Dispatch<false>(tw, b->lds_levels, a);
+  const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");
   RT_HIP_CHECK(hipGetLastError());
   if (count) RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_stats, 8 * 4, hipMemcpyDeviceToHost, s));
@@ -478,7 +486,7 @@ int rt_batch_config(rt_batch* b, int tile_shape, int lds_levels, int wgs_per_cu,
   RT_REQUIRE(lds_levels == 8 || lds_levels == 10 || lds_levels == 12 || lds_levels == 16, RT_ERR_RANGE,
              "lds_levels must be 8, 10, 12 or 16");
   RT_REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= 32, RT_ERR_RANGE, "wgs_per_cu must be in [1, 32]");
-  RT_REQUIRE(tree_depth >= 1 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [1, 22]");
+  RT_REQUIRE(tree_depth >= 0 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [0, 22] (0: unknown)");
   b->tile_shape = tile_shape;
   b->lds_levels = lds_levels;
   b->wgs_per_cu = wgs_per_cu;

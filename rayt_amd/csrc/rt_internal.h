@@ -20,6 +20,7 @@ struct rt_ctx {
   int plain_lds_levels = 16;  // LDS stack levels for the plain kernel
   int plain_waves = 4;        // waves per workgroup of the plain kernel (1: 8x8 pixels, 4: 16x16)
   int plain_lds_base = 0;     // first stack level held in LDS
+  bool plain_spill = true;    // false: the declared tree depth fits the LDS window (window-only stack)
 };
 
 struct rt_buf {

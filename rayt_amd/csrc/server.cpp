@@ -418,7 +418,7 @@ int rt_server_create(rt_ctx* ctx, const rt_server_config* cfg, rt_server** out) 
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       rc = rayt::Fail(RT_ERR_HIP, "server stream");
     if (rc == RT_OK) rc = rayt::BatchCreate(ctx, cfg->max_views_per_round, &sl.batch);
-    if (rc == RT_OK) rc = rt_batch_config(sl.batch, cfg->tile_shape, cfg->lds_levels, cfg->wgs_per_cu, 11);
+    if (rc == RT_OK) rc = rt_batch_config(sl.batch, cfg->tile_shape, cfg->lds_levels, cfg->wgs_per_cu, 0);
   }
   if (rc != RT_OK) {
     rt_server_destroy(s);

@@ -56,15 +56,28 @@ struct StackSpill {
   uint2 v[LDS_LEVELS >= kStackLevels ? 1 : kStackLevels];
 };
 
-template <int LDS_LEVELS, int BLOCK>
+//
+// SPILL = false is the window-only stack, for trees the caller declares to
+// fit the window (depth - 1 <= LDS_LEVELS, base 0): no private array and no
+// branch per access. A level outside the window is never stored out of
+// bounds (the index is clamped) and `put` reports it, and the traversal then
+// ends the ray with the iteration-limit error, so a tree deeper than declared
+// shows error pixels instead of wrong ones.
+template <int LDS_LEVELS, int BLOCK, bool SPILL = true>
 struct SplitStack {
   uint2* lds;             // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
   int base = 0;           // first level held in LDS
-  uint2* spill = nullptr;  // StackSpill<LDS_LEVELS>::v of this thread
-  __device__ __forceinline__ void put(int scale, uint2 v) {
+  uint2* spill = nullptr;  // StackSpill<LDS_LEVELS>::v of this thread (SPILL only)
+  // Returns false when a window-only stack could not store the level.
+  __device__ __forceinline__ bool put(int scale, uint2 v) {
     const int level = (kSMax - 1) - scale;
     if constexpr (LDS_LEVELS >= kStackLevels) {
       lds[level * BLOCK] = v;
+      return true;
+    } else if constexpr (!SPILL) {
+      const unsigned l = unsigned(level - base);
+      lds[min(l, unsigned(LDS_LEVELS - 1)) * BLOCK] = v;
+      return l < unsigned(LDS_LEVELS);
     } else {
       const int l = level - base;
       if (unsigned(l) < unsigned(LDS_LEVELS)) {
@@ -75,12 +88,15 @@ struct SplitStack {
         // through a selected pointer instead of a ds_write
         __asm__ volatile("" ::: "memory");
       }
+      return true;
     }
   }
   __device__ __forceinline__ uint2 get(int scale) const {
     const int level = (kSMax - 1) - scale;
     if constexpr (LDS_LEVELS >= kStackLevels) {
       return lds[level * BLOCK];
+    } else if constexpr (!SPILL) {
+      return lds[min(unsigned(level - base), unsigned(LDS_LEVELS - 1)) * BLOCK];
     } else {
       const int l = level - base;
       uint2 r;
@@ -232,7 +248,8 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     const float tx_center = half * s.tx_coef + tx_corner;
     const float ty_center = half * s.ty_coef + ty_corner;
     const float tz_center = half * s.tz_coef + tz_corner;
-    if (tc_max < s.h) stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)));
+    if (tc_max < s.h && !stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max))))
+      s.iter = kIterLimitSteps;  // deeper than a window-only stack holds: the next step reports an error
     s.h = tc_max;
     s.parent = ofs;
     s.scale--;

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 // WAVES = 4: workgroups of 256 lanes over 16x16 pixels (four 8x8 wave tiles);
 // WAVES = 1: one wave per workgroup over an 8x8 tile, so a finished wave frees
 // its LDS stack at once and more waves fit a CU.
-template <int WAVES, int LDS_LEVELS, bool COUNT>
+template <int WAVES, int LDS_LEVELS, bool COUNT, bool SPILL>
 __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     uint32_t* __restrict__ out, int* __restrict__ codes, int W, int H, PoolView pool, uint32_t root,
     const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors, int nodes_in_block, Mat16 m,
@@ -176,15 +176,15 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
   PixelCounters cnt;
   if (x < W && y < H) {
     const int64_t i = int64_t(y) * W + x;
-    StackSpill<LDS_LEVELS> spill;
-    SplitStack<LDS_LEVELS, BLOCK> stack;
+    StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
+    SplitStack<LDS_LEVELS, BLOCK, SPILL> stack;
     stack.spill = spill.v;
     stack.lds = lds + threadIdx.x;
     stack.base = lds_base;
     uint32_t color;
     int code;
-    RenderPixel<SplitStack<LDS_LEVELS, BLOCK>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y, W, H,
-                                                      lod, bg.v, flags, lt, stack, color, code, cnt);
+    RenderPixel<SplitStack<LDS_LEVELS, BLOCK, SPILL>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y,
+                                                             W, H, lod, bg.v, flags, lt, stack, color, code, cnt);
     out[i] = color;
     if (codes) codes[i] = code;
   }
@@ -325,25 +325,31 @@ void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, in
                                                   lt);
 }
 
-template <int WV, int L, bool C>
+template <int WV, int L, bool C, bool SP>
 void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
                   uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib, const Mat16& m, float lod,
                   const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st) {
-  RenderPlainKernel<WV, L, C><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg,
-                                                          flags, lt, base, st);
+  RenderPlainKernel<WV, L, C, SP><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod,
+                                                              bg, flags, lt, base, st);
 }
 
+// spill = false selects the window-only stack (the tree fits the LDS window).
 template <bool C>
-bool DispatchPlain(int waves, int levels, hipStream_t s, uint32_t* out, int* codes, int W, int H,
+bool DispatchPlain(int waves, int levels, bool spill, hipStream_t s, uint32_t* out, int* codes, int W, int H,
                    const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
                    const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, int base,
                    unsigned long long* st) {
   const int ts = waves == 4 ? 16 : 8;
   const dim3 g(uint32_t((W + ts - 1) / ts) * uint32_t((H + ts - 1) / ts));
-#define RT_CASE(WV, L)                                                                                       \
-  if (waves == WV && levels == L) {                                                                          \
-    LaunchPlainL<WV, L, C>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, base, st); \
-    return true;                                                                                             \
+#define RT_CASE(WV, L)                                                                                      \
+  if (waves == WV && levels == L) {                                                                         \
+    if (spill || L >= kStackLevels)                                                                         \
+      LaunchPlainL<WV, L, C, true>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, \
+                                   base, st);                                                               \
+    else                                                                                                    \
+      LaunchPlainL<WV, L, C, L >= kStackLevels>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod,  \
+                                                bg, flags, lt, base, st);                                   \
+    return true;                                                                                            \
   }
   RT_CASE(1, 8) RT_CASE(1, 10) RT_CASE(1, 12) RT_CASE(1, 14) RT_CASE(1, 16) RT_CASE(1, 22)
   RT_CASE(4, 8) RT_CASE(4, 10) RT_CASE(4, 12) RT_CASE(4, 14) RT_CASE(4, 16) RT_CASE(4, 22)
@@ -476,6 +482,8 @@ int rt_plain_config(rt_ctx* ctx, int lds_stack_levels, int wg_waves, int tree_de
   ctx->plain_waves = wg_waves;
   // A depth-D tree pushes stack levels 0 .. D-2; keep the deepest ones in LDS.
   ctx->plain_lds_base = tree_depth > 0 ? std::max(0, (tree_depth - 1) - lds_stack_levels) : 0;
+  // A declared depth that fits the window selects the window-only stack.
+  ctx->plain_spill = !(tree_depth > 0 && tree_depth - 1 <= lds_stack_levels);
   return RT_OK;
 }
 
@@ -501,7 +509,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
     unsigned long long* d_stats = nullptr;
     RT_HIP_CHECK(hipMalloc(&d_stats, 4 * sizeof(unsigned long long)));
     hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream);
-    DispatchPlain<true>(ctx->plain_waves, ctx->plain_lds_levels, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+    DispatchPlain<true>(ctx->plain_waves, ctx->plain_lds_levels, ctx->plain_spill, ctx->stream,
+                        static_cast<uint32_t*>(out_image->ptr),
                         codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                         uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
                         lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, d_stats);
@@ -514,7 +523,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
     for (int k = 0; k < 4; ++k) stats[k] = h[k];
     return RT_OK;
   }
-  DispatchPlain<false>(ctx->plain_waves, ctx->plain_lds_levels, ctx->stream, static_cast<uint32_t*>(out_image->ptr),
+  DispatchPlain<false>(ctx->plain_waves, ctx->plain_lds_levels, ctx->plain_spill, ctx->stream,
+                       static_cast<uint32_t*>(out_image->ptr),
                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                        uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
                        lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr);

@@ -58,6 +58,26 @@ def test_batch_matches_plain(gpu_ctx, scene, tile_shape, lds_levels):
         np.testing.assert_array_equal(cod[off:off + w * h], pc, err_msg=f"view {w}x{h} codes")
 
 
+@pytest.mark.parametrize("lds_levels,tree_depth", [(8, 0), (8, 14), (12, 0)])
+def test_batch_spill_stack_matches_plain(gpu_ctx, scene, lds_levels, tree_depth):
+    """The spill stack (depth unknown, or declared deeper than the LDS window holds) gives the same bytes."""
+    m = scene
+    specs = _views(8, seed=lds_levels + tree_depth)
+    flags = api.L.RT_FLAG_SHADOWS
+    b = api.BatchRenderer(m, 16)
+    b.config(tile_shape=2, lds_levels=lds_levels, wgs_per_cu=16, tree_depth=tree_depth)
+    views, total, offsets = b.make_views(specs, flags=flags)
+    out = api.Buffer(gpu_ctx, 4 * total)
+    codes = api.Buffer(gpu_ctx, 4 * total)
+    b.render(views, len(specs), out, codes)
+    img = out.read_all(np.uint32)
+    cod = codes.read_all(np.int32)
+    for (vpi, w, h), off in zip(specs, offsets):
+        pi, pc = _plain(gpu_ctx, m, vpi, w, h, flags)
+        np.testing.assert_array_equal(img[off:off + w * h], pi, err_msg=f"view {w}x{h} image")
+        np.testing.assert_array_equal(cod[off:off + w * h], pc, err_msg=f"view {w}x{h} codes")
+
+
 def test_batch_counts_add_up(gpu_ctx, scene):
     m = scene
     specs = _views(9, seed=3)

@@ -85,7 +85,10 @@ def test_render_plain_matches_oracle(gpu_ctx, lod, flags):
         np.testing.assert_array_equal(img, r["image"], err_msg=f"image differs at pose {yaw, pitch, pos}")
 
 
-@pytest.mark.parametrize("waves,levels,depth", [(1, 8, 8), (1, 10, 0), (4, 8, 8), (1, 22, 8), (4, 14, 8)])
+# (waves, LDS levels, declared depth): window-only stacks (depth - 1 <= levels), spill stacks with the
+# window at level 0 (depth unknown) and at level 3 (declared 12 over 8 levels), and the full LDS stack.
+@pytest.mark.parametrize("waves,levels,depth", [(1, 8, 8), (1, 10, 0), (4, 8, 8), (1, 22, 8), (4, 14, 8),
+                                                (4, 8, 12), (4, 16, 0)])
 def test_render_plain_launch_shapes(gpu_ctx, waves, levels, depth):
     """Every launch shape (workgroup size, LDS stack window, spill to private memory) gives the same bytes."""
     tree, path = terrain_tree(8, 8192, 8, 0)
@@ -102,6 +105,34 @@ def test_render_plain_launch_shapes(gpu_ctx, waves, levels, depth):
                                  flags=api.L.RT_FLAG_SHADOWS)
             np.testing.assert_array_equal(out.read_all(np.uint32).reshape(H, W), r["image"])
             np.testing.assert_array_equal(codes.read_all(np.int32).reshape(H, W), r["codes"])
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+
+
+def test_render_plain_declared_depth_too_small(gpu_ctx):
+    """A window-only stack on a tree deeper than declared ends the deeper rays as error pixels (red, no-hit
+    code) and leaves every other pixel as the spill stack renders it."""
+    tree, _ = terrain_tree(10, 8192, 0, 16)
+    m = _gpu_cache(gpu_ctx, tree)
+    W, H = 320, 240
+    red, no_hit = 0xFF0000FF, 0x7FFFFFFF
+    try:
+        errors = 0
+        for yaw, pitch, pos in POSES[:4]:
+            vpi = camera_vpi(yaw, pitch, pos)
+            api.plain_config(gpu_ctx, 16, 4, 0)
+            out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi)
+            ref_img, ref_codes = out.read_all(np.uint32), codes.read_all(np.int32)
+            assert not np.any(ref_img == red)
+            api.plain_config(gpu_ctx, 8, 4, 9)  # window of 8 levels; the tree pushes up to level 8
+            out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi)
+            img, cod = out.read_all(np.uint32), codes.read_all(np.int32)
+            bad = img == red
+            assert np.all(cod[bad] == no_hit)
+            np.testing.assert_array_equal(img[~bad], ref_img[~bad])
+            np.testing.assert_array_equal(cod[~bad], ref_codes[~bad])
+            errors += int(bad.sum())
+        assert errors > 0
     finally:
         api.plain_config(gpu_ctx, 16, 4, 0)
 
