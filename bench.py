@@ -140,7 +140,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     flags = api.L.RT_FLAG_SHADOWS
     rng = np.random.default_rng(seed)
     srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
-                           pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=16)
+                           pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=24)
     for tree, depth in scenes:
         srv.add_scene(tree, depth)
 
@@ -160,6 +160,8 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     got = {c.id: (img, cod) for c, img, cod in srv.poll(4096)}
     caches = {}
     bad = 0
+    # The served scenes go deeper than the frame's: the plain reference renders with a stack of unknown depth.
+    api.plain_config(ctx, 16, 4, 0)
     for rid, (vpi, w, h, s) in zip(ids, chk):
         if s not in caches:
             tree, _ = scenes[s]

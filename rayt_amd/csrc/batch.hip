@@ -9,9 +9,12 @@
 // are single waves. A wave claims groups of tiles from its XCD's range with
 // one atomic and, when that range is exhausted, steals from the next XCD's;
 // after all eight ranges are exhausted every wave exits, so the grid drains.
-// Inside a wave, a lane whose ray (primary, then shadow) has finished pulls
-// the next pixel of the wave's tiles, so lanes stay busy while the slowest
-// ray of a tile is still being traced. The traversal stack keeps the deepest
+// Inside a wave, lanes whose ray (primary, then shadow) has finished wait
+// until kSettleMin lanes have; then all of them are completed together
+// (shading, the shadow ray's start, the stores) and the idle ones take the
+// next pixels of the wave's tiles together, so this per-ray code runs once
+// for a group of lanes. Shadow rays of one group are traced beside the
+// primary rays of the next. The traversal stack keeps the deepest
 // `lds_levels` levels in LDS.
 #include <algorithm>
 #include <cstring>
@@ -26,10 +29,14 @@ using namespace rtdev;
 
 namespace {
 
-constexpr uint32_t kTilesPerGrab = 4;
-// A refill happens when at least this many lanes are idle, so its
-// bookkeeping is paid once per several traversal steps.
-constexpr int kRefillMin = 16;
+// Tiles claimed by one atomic: up to this many, fewer when a launch has
+// too few tiles per wave for claims of this size to balance the load.
+constexpr uint32_t kMaxTilesPerGrab = 4;
+// Completion and refill run when at least this many lanes have finished
+// their ray (or when no lane is tracing), so their code is executed once
+// per group of lanes rather than once per ray. All 64 measured fastest
+// (against 16, 32 and 48) on a 64-view mix and on a single 1080p view.
+constexpr int kSettleMin = 64;
 
 struct DevScene {
   const uint32_t* links;
@@ -56,13 +63,13 @@ struct Vec4B {
   float v[4];
 };
 
+// At most 80 VGPRs (six waves per SIMD): the traversal needs the latency hiding more than the
+// refill and shading code needs registers.
 template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL>
-__global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ views, int nviews,
-                                                  const DevScene* __restrict__ scenes,
-                                                  const uint32_t* __restrict__ tile_begin, uint32_t total_tiles,
-                                                  uint32_t* __restrict__ counters, uint32_t* __restrict__ out,
-                                                  int* __restrict__ codes, Vec4B bg, Lights lt,
-                                                  unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void BatchKernel(
+    const DevView* __restrict__ views, int nviews, const DevScene* __restrict__ scenes,
+    const uint32_t* __restrict__ tile_begin, uint32_t total_tiles, uint32_t grab, uint32_t* __restrict__ counters,
+    uint32_t* __restrict__ out, int* __restrict__ codes, Vec4B bg, Lights lt, unsigned long long* __restrict__ stats) {
   constexpr int TH = 64 / TW;
   __shared__ uint2 lds[LDS_LEVELS * 64];
   StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
@@ -81,7 +88,9 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
   bool queues_done = false;
 
   // lane state
-  bool active = false;
+  bool has_ray = false;  // the lane holds a pixel
+  bool tracing = false;  // ... whose current ray needs more steps
+  int status = kRunning;
   int phase = 0;  // 0: primary ray, 1: shadow ray
   Trav tr;
   PoolView pool;
@@ -97,8 +106,8 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
   float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
 
   while (true) {
-    uint64_t need = __ballot(!active);
-    if (!queues_done && __popcll(need) >= kRefillMin) {
+    uint64_t need = __ballot(!has_ray);
+    if (!queues_done && need) {
       while (need && !queues_done) {
         if (chunk_pix >= chunk_npix) {
           while (true) {  // claim the next chunk from this XCD's queue, then steal from the others
@@ -109,11 +118,11 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
             const uint32_t q = (xcd + q_rel) & 7u;
             const uint32_t qbegin = min(total_tiles, q * per), qend = min(total_tiles, qbegin + per);
             uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(&counters[q], kTilesPerGrab);
+            if (lane == 0) t = atomicAdd(&counters[q], grab);
             t = __shfl(t, 0, 64);
             if (t < per && qbegin + t < qend) {
               chunk_first = qbegin + t;
-              chunk_npix = (min(qend, chunk_first + kTilesPerGrab) - chunk_first) * 64u;
+              chunk_npix = (min(qend, chunk_first + grab) - chunk_first) * 64u;
               chunk_pix = 0;
               break;
             }
@@ -124,7 +133,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
         const uint32_t avail = chunk_npix - chunk_pix;
         const uint32_t rank = uint32_t(__popcll(need & lanes_below));
         const uint32_t take = min(uint32_t(__popcll(need)), avail);
-        if (!active && rank < take) {
+        if (!has_ray && rank < take) {
           const uint32_t p = chunk_pix + rank;
           const uint32_t tile = chunk_first + p / 64u, local_px = p % 64u;
           int lo = 0, hi = nviews - 1;  // view holding the tile: last v with tile_begin[v] <= tile
@@ -154,21 +163,29 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
             pix = v.pixel_offset + uint64_t(y) * v.width + x;
             flags = v.flags;
             phase = 0;
-            active = true;
+            has_ray = tracing = true;
             if (COUNT) ++cnt.primaries;
           }
         }
         chunk_pix += take;
-        need = __ballot(!active);
+        need = __ballot(!has_ray);
       }
     }
-    if (!__any(active)) {
+    if (!__any(has_ray)) {
       if (queues_done) break;
       continue;  // every claimed pixel so far was outside its view; claim more
     }
-    if (!active) continue;
-    const int status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT>(tr, pool, stack, fetches);
-    if (status == kRunning) continue;
+    // Trace until kSettleMin lanes have finished (idle lanes count while
+    // pixels remain to be claimed) or no lane is tracing.
+    while (true) {
+      if (tracing) {
+        status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT>(tr, pool, stack, fetches);
+        tracing = status == kRunning;
+      }
+      if (!__any(tracing)) break;
+      if (__popcll(__ballot(!tracing && (has_ray || !queues_done))) >= kSettleMin) break;
+    }
+    if (!has_ray || tracing) continue;
     CastResult res;
     TravResult(tr, status, res);
     uint32_t color;
@@ -200,6 +217,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
           TravInit(tr, pool, root, ox + dx * tb, oy + dy * tb, oz + dz * tb, lt.shadow_dir[0], lt.shadow_dir[1],
                    lt.shadow_dir[2], 0.0f, 0.0f);
           phase = 1;
+          tracing = true;
           if (COUNT) ++cnt.shadow_rays;
           continue;
         }
@@ -211,7 +229,7 @@ __global__ __launch_bounds__(64) void BatchKernel(const DevView* __restrict__ vi
     }
     out[pix] = Blend(color, bg.v);
     if (codes) codes[pix] = code;
-    active = false;
+    has_ray = false;
   }
   if (COUNT) {
     cnt.fetches = fetches;
@@ -231,6 +249,7 @@ struct LaunchArgs {
   const DevScene* scenes;
   const uint32_t* tile_begin;
   uint32_t total_tiles;
+  uint32_t grab;
   uint32_t* counters;
   uint32_t* out;
   int* codes;
@@ -242,7 +261,7 @@ struct LaunchArgs {
 template <int TW, int L, bool C, bool SP>
 void Launch(const LaunchArgs& a) {
   BatchKernel<TW, L, C, SP><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles,
-                                                           a.counters, a.out, a.codes, a.bg, a.lt, a.stats);
+                                                           a.grab, a.counters, a.out, a.codes, a.bg, a.lt, a.stats);
 }
 
 // spill = false selects the window-only stack (every scene fits the LDS window).
@@ -413,10 +432,12 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   const float kBg[4] = {0.0f, float(216 / 255.), 1.0f, 1.0f};
   memcpy(a.bg.v, kBg, sizeof(a.bg.v));
   a.lt = MakeLights();
-  // Enough single-wave workgroups for the work, at most wgs_per_cu per CU, a multiple of 8 (one range per XCD).
-  const uint64_t want = std::min<uint64_t>(uint64_t(std::max(1, b->ctx->compute_units)) * b->wgs_per_cu,
-                                           (tiles + kTilesPerGrab - 1) / kTilesPerGrab);
-  a.grid = dim3(uint32_t((std::max<uint64_t>(want, 1) + 7) & ~uint64_t(7)));
+  // Enough single-wave workgroups for the work, at most wgs_per_cu per CU, a multiple of 8 (one range per XCD);
+  // claims of several tiles only when every wave gets at least 16 of them.
+  const uint64_t want = std::min<uint64_t>(uint64_t(std::max(1, b->ctx->compute_units)) * b->wgs_per_cu, tiles);
+  const uint64_t waves = (std::max<uint64_t>(want, 1) + 7) & ~uint64_t(7);
+  a.grid = dim3(uint32_t(waves));
+  a.grab = uint32_t(std::clamp<uint64_t>(tiles / (waves * 16), 1, kMaxTilesPerGrab));
   a.stats = count ? b->d_stats : nullptr;
   RT_HIP_CHECK(hipEventRecord(b->ev0, s));
   const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
@@ -517,7 +538,8 @@ int rt_batch_autotune(rt_batch* b, const rt_view* views, int nviews, rt_buf* out
   RT_REQUIRE(repeats >= 1 && repeats <= 16, RT_ERR_RANGE, "repeats must be in [1, 16]");
   static const int kShapes[3] = {0, 1, 2};
   static const int kLevels[4] = {8, 10, 12, 16};
-  static const int kWgs[4] = {8, 16, 24, 32};
+  // The kernel is built for at most six waves per SIMD: more workgroups per CU than 24 would wait.
+  static const int kWgs[4] = {12, 16, 20, 24};
   double best_t = 1e300;
   int bc[3] = {b->tile_shape, b->lds_levels, b->wgs_per_cu};
   for (int shape : kShapes)

@@ -119,6 +119,16 @@ struct PoolView {
   const uint32_t* __restrict__ far_ptrs;
   uint32_t nodes;
   uint32_t far_count;
+  // Loads through global-address-space pointers: where the pool pointers are
+  // read from memory (one pool per view) the compiler cannot tell their
+  // address space and would emit FLAT loads, which also count against the
+  // LDS wait counter and so stall the stack reads behind node fetches.
+  __device__ __forceinline__ uint32_t link(uint32_t i) const {
+    return ((const __attribute__((address_space(1))) uint32_t*)links)[i];
+  }
+  __device__ __forceinline__ uint32_t far(uint32_t i) const {
+    return ((const __attribute__((address_space(1))) uint32_t*)far_ptrs)[i];
+  }
 };
 
 // ESVO traversal over the cache link format, as a resumable state machine:
@@ -170,7 +180,7 @@ __device__ __forceinline__ void TravInit(Trav& s, const PoolView& pool, uint32_t
   s.h = s.t_max;
   s.t_min = cl_fmax(s.t_min, 0.0f);
   s.parent = root_node;
-  s.child_descriptor = pool.links[root_node];
+  s.child_descriptor = pool.link(root_node);
   s.pending_fetch = 1;
   s.idx = 0;
   s.posx = s.posy = s.posz = 1.0f;
@@ -226,7 +236,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   const bool far_bad = is_far && ptr >= pool.far_count;
   uint32_t ofs = ptr + s.parent - (1u << 20);
   if (is_far && !far_bad) {
-    ofs = pool.far_ptrs[ptr];
+    ofs = pool.far(ptr);
     if (COUNT) ++fetches;
   }
   ofs += __popc(desc & (child_masks - 1u) & 255u);
@@ -234,7 +244,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   const bool fetch = want && !fault && !bad;
   uint32_t child = desc;
   if (fetch) {
-    child = pool.links[ofs];
+    child = pool.link(ofs);
     if (COUNT) ++fetches;
   }
   const bool hit = fetch && (!(child & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2);
@@ -292,7 +302,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
         s.posz = __uint_as_float(shz << scale);
         s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
         s.h = 0.0f;
-        s.child_descriptor = pool.links[s.parent];
+        s.child_descriptor = pool.link(s.parent);
         s.pending_fetch = 1;
       }
     }
