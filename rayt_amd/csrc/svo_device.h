@@ -209,10 +209,11 @@ enum TravStatus : int {
 // only the three node loads (the parent descriptor after a pop, a far
 // pointer, the child descriptor) and the stack accesses are guarded. Every
 // way out of the loop only sets the status, which TravResult decodes once, so
-// the loop around the step has a single exit.
+// the loop around the step has a single exit. The reference's loop test
+// (scale < S_MAX) is evaluated where the scale grows, at the end of a pop,
+// and ends the ray in that step.
 template <class Stack, bool COUNT>
 __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& stack, uint32_t& fetches) {
-  if (s.scale >= kSMax) return kMiss;
   if (++s.iter > kIterLimitSteps) return kIterLimit;
   if (COUNT) {
     fetches += (s.pending_fetch || s.child_descriptor == 0) ? 1u : 0u;
@@ -249,7 +250,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   }
   const bool hit = fetch && (!(child & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2);
   const bool push = fetch && !hit;
-  const int status = fault ? int(kFault) : bad ? int(kBadLink) : hit ? int(kHit) : int(kRunning);
+  int status = fault ? int(kFault) : bad ? int(kBadLink) : hit ? int(kHit) : int(kRunning);
   s.child_descriptor = child;
   if (hit) s.parent = ofs;
 
@@ -289,7 +290,9 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
       const uint32_t dz = sz ? __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2) : 0u;
       const int scale = 31 - int(__clz(dx | dy | dz));
       s.scale = scale;
-      if (scale < kSMax) {  // otherwise the ray left the root cube; the next step reports the miss
+      if (scale >= kSMax) {
+        status = kMiss;  // the ray left the root cube: the loop test (raycast.cl:250, 254-255)
+      } else {
         s.scale_exp2 = __uint_as_float(uint32_t(scale - kSMax + 127) << 23);
         const uint2 e = stack.get(scale);
         s.parent = e.x;
