@@ -265,8 +265,9 @@ def spline_poses(n: int, seed: int):
 
 
 def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, check_frames: int = 2,
-                        lod: float = 1.0):
-    """Fly-through over a block pool that holds `frac` of the tree's blocks: every frame runs passes, feedback
+                        lod: float = 1.0, pool_gb: float = 0.0):
+    """Fly-through over a block pool that holds `frac` of the tree's blocks (or `pool_gb` GiB of blocks when
+    given): every frame runs passes, feedback
     and cache transactions (faulted blocks uploaded from host memory under LRU eviction) until no ray faults
     or the session fills the pool. LOD voxel size 1 (the application's default). The first `check_frames`
     frames are compared with the oracle (images and per-pass fault lists)."""
@@ -275,7 +276,11 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     info = tree.info()
     # The fused single-pass frames run the plain kernel: declare this tree's depth to it.
     api.plain_config(ctx, 16, 4, depth)
-    max_blocks = max(8, int(info.blocks_count * frac))
+    block_bytes = info.bytes_in_block_content + 36  # channels + far-pointer slots + page-table entry
+    if pool_gb > 0:
+        max_blocks = max(8, min(info.blocks_count, int(pool_gb * (1 << 30) / block_bytes)))
+    else:
+        max_blocks = max(8, int(info.blocks_count * frac))
     cache = api.GPUOctreeCacheManager(max_blocks, tree, ctx)
     cache.InitialFillCache()
     tr = api.GPURayTracer(cache, W, H)
@@ -306,6 +311,8 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
         stt = tr.frame_stats()
         passes += stt[0]
         rays += stt[4]
+        if (k + 1) % 32 == 0:
+            log(f"out-of-core: {k + 1} frames")
         if oc is not None and k < check_frames:
             a = time.perf_counter()
             r = oc.render_frame(W, H, vpi, lod=lod, flags=api.L.RT_FLAG_SHADOWS, seed=rng, expected_faults=ef)
@@ -321,7 +328,7 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
         del oc, ot
         os.remove(path)
     res = {"depth": depth, "blocks": info.blocks_count, "pool_blocks": max_blocks, "lod_voxel_size": lod,
-           "pool_bytes": max_blocks * (info.bytes_in_block_content + 36), "frames": frames,
+           "pool_bytes": max_blocks * block_bytes, "frames": frames,
            "width": W, "height": H, "frames_per_s": round(frames / wall, 2), "mrays_s": round(rays / wall / 1e6, 1),
            "passes_per_frame": round(passes / frames, 2), "uploads_per_frame": round(uploads / frames, 1),
            "frames_checked_bit_exact": checked}
@@ -400,6 +407,8 @@ def main():
     ap.add_argument("--no-outofcore", action="store_true")
     ap.add_argument("--ooc-depth", type=int, default=12)
     ap.add_argument("--ooc-frac", type=float, default=0.25, help="pool size as a fraction of the tree's blocks")
+    ap.add_argument("--ooc-pool-gb", type=float, default=0.0, help="pool size in GiB (instead of --ooc-frac)")
+    ap.add_argument("--ooc-check-frames", type=int, default=2, help="frames compared with the oracle")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
     ap.add_argument("--dump", default="", metavar="DIR", help="write the measured frame and its feedback codes as PNG")
@@ -520,10 +529,13 @@ def main():
     if not args.no_outofcore:
         t = time.time()
         ospec = api.TerrainSpec(depth=args.ooc_depth, seed=14, octaves=6)
+        if args.ooc_depth >= 13:
+            os.environ["RAYT_BUILD_LOG"] = "1"  # the builder reports its phases: minutes of work
         otree = api.StoredOctree.terrain(ospec, 8192)
         log(f"out-of-core scene depth {args.ooc_depth} (6 octaves): {otree.info().blocks_count} blocks, "
             f"built in {time.time() - t:.1f}s")
-        outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H)
+        outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H,
+                                        check_frames=args.ooc_check_frames, pool_gb=args.ooc_pool_gb)
         del otree
     cpu_plain = None if args.no_cpu_plain or args.no_cpu_baseline else \
         measure_cpu_plain_config(ctx, max(1, min(16, os.cpu_count() or 1)))

@@ -6,6 +6,8 @@
 #include "builder.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <optional>
 #include <queue>
 #include <stdexcept>
@@ -78,6 +80,8 @@ class Builder {
   const NodeTree& t_;
   int nib_;
   MemoryTree* out_;
+  bool log_ = std::getenv("RAYT_BUILD_LOG") != nullptr;
+  int committed_ = 0;
   std::vector<uint8_t> payload_;
   std::vector<size_t> ch_offsets_;
   StoredOctreeBlockHeader hdr_;
@@ -145,6 +149,10 @@ class Builder {
   }
 
   void CommitBlock(const std::vector<Subtree>& subtrees, int block_index, uint32_t parent_block_index) {
+    if (log_ && ++committed_ % 16384 == 0) {  // progress of long builds (RAYT_BUILD_LOG)
+      std::fprintf(stderr, "[rayt build] %d blocks committed\n", committed_);
+      std::fflush(stderr);
+    }
     if (subtrees.empty() || subtrees.size() > 8) throw std::logic_error("block with 0 or >8 roots");
     if (uint32_t(block_index) >= kMaxLinkPtr) throw std::runtime_error("more than 2^21 blocks");
     int node_count = 0;

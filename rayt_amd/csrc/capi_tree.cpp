@@ -1,4 +1,7 @@
 // C ABI over block sources: .tree files and trees built in host RAM.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -10,11 +13,23 @@ using rayt::BlockSource;
 
 namespace {
 
+using BuildClock = std::chrono::steady_clock;
+
+// With RAYT_BUILD_LOG set, large builds report their phases on stderr.
+void BuildLog(const char* what, uint64_t count, BuildClock::time_point t0) {
+  if (!std::getenv("RAYT_BUILD_LOG")) return;
+  const double s = std::chrono::duration<double>(BuildClock::now() - t0).count();
+  std::fprintf(stderr, "[rayt build] %s: %llu in %.1f s\n", what, static_cast<unsigned long long>(count), s);
+  std::fflush(stderr);
+}
+
 int FinishBuild(const rayt::NodeTree& t, int nodes_in_block, rt_tree** out) {
   std::string err;
   rayt::BuildReport rep;
+  const auto t0 = BuildClock::now();
   std::unique_ptr<rayt::MemoryTree> mt = rayt::BuildOctree(t, nodes_in_block, &rep, &err);
   if (!mt) return rayt::Fail(RT_ERR_INVALID, "build failed: " + err);
+  BuildLog("blocks", rep.blocks, t0);
   if (rep.allocated_groups != rep.deallocated_groups)
     return rayt::Fail(RT_ERR_STATE, "builder leaked sibling groups");
   rt_tree* h = new rt_tree();
@@ -69,7 +84,9 @@ int rt_tree_build_terrain(const rt_terrain_params* p, int nodes_in_block, rt_tre
   tp.box_emin = p->box_emin;
   tp.box_emax = p->box_emax;
   tp.threads = p->threads;
+  const auto t0 = BuildClock::now();
   rayt::NodeTree t = rayt::GenerateTerrainTree(tp);
+  BuildLog("terrain nodes", t.size(), t0);
   return FinishBuild(t, nodes_in_block, out);
   RT_TRY_END
 }

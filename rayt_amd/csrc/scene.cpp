@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 
 namespace rayt {
@@ -379,6 +381,10 @@ NodeTree TerrainBuilder::Build() {
     for (auto& pp : parts) total += pp.size();
     levels[L + 1].reserve(total);
     for (auto& pp : parts) levels[L + 1].insert(levels[L + 1].end(), pp.begin(), pp.end());
+    if (std::getenv("RAYT_BUILD_LOG")) {  // progress of long builds
+      std::fprintf(stderr, "[rayt build] level %d: %zu cells\n", L + 1, total);
+      std::fflush(stderr);
+    }
   }
   masks[p_.depth].assign(levels[p_.depth].size(), 0);
 
