@@ -128,21 +128,45 @@ def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: 
     return res
 
 
-def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16):
-    """Poisson request stream over resident scenes through the native continuous-batching server.
+def pool_shares(trees, budget: int):
+    """Blocks of pool per scene under a shared byte budget, by water-filling: scenes smaller than an equal
+    share of what is left are fully resident, the others split the rest equally (served out of core)."""
+    sizes = [(t.info().blocks_count, t.info().bytes_in_block_content + 36) for t in trees]
+    order = sorted(range(len(trees)), key=lambda i: sizes[i][0] * sizes[i][1])
+    left, out = budget, [0] * len(trees)
+    for k, i in enumerate(order):
+        blocks, bb = sizes[i]
+        share = left // (len(trees) - k)
+        out[i] = min(blocks, share // bb)
+        left -= out[i] * bb
+    return out
 
-    scenes: list of (tree, depth). Each request is a seeded pose (position uniform in [0,1]^3, direction
-    uniform on the sphere) and a resolution from the batched mix, on a uniformly drawn scene. Returns the
-    open-loop run (p50/p99 latency at the offered rate) and a closed burst (all requests queued at once:
-    the server's throughput at saturation)."""
+
+def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16,
+                   pool_gb: float = 16.0, max_block_loads: int = 128):
+    """Poisson request stream over several scenes through the native continuous-batching server.
+
+    scenes: list of (tree, depth), sharing a block-pool budget of `pool_gb` GiB (pool_shares: scenes that
+    do not fit are served out of core, loading blocks from the feedback of their requests). Each request
+    is a seeded pose (position uniform in [0,1]^3, direction uniform on the sphere) and a resolution from
+    the batched mix, on a uniformly drawn scene. Returns the open-loop run (p50/p99 latency at the offered
+    rate) and a closed burst (all requests queued at once: the server's throughput at saturation).
+    max_block_loads bounds the blocks loaded per out-of-core scene between rounds."""
     import numpy as np
     from rayt_amd import api
     flags = api.L.RT_FLAG_SHADOWS
     rng = np.random.default_rng(seed)
+    budget = int(pool_gb * (1 << 30))
     srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
-                           pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=24)
-    for tree, depth in scenes:
-        srv.add_scene(tree, depth)
+                           pool_bytes=budget, tile_shape=0, lds_levels=12, wgs_per_cu=24,
+                           max_block_loads=max_block_loads)
+    shares = pool_shares([t for t, _ in scenes], budget)
+    partial = []
+    for (tree, depth), share in zip(scenes, shares):
+        srv.add_scene(tree, depth, max_blocks=share)
+        partial.append(share < tree.info().blocks_count)
+    pools = [{"depth": d, "blocks": t.info().blocks_count, "pool_blocks": sh} for (t, d), sh in zip(scenes, shares)]
+    log(f"served scene pools: {pools}")
 
     def make(n):
         specs = view_mix(n, int(rng.integers(1 << 30)))
@@ -157,18 +181,22 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     chk = make(check)
     ids = srv.submit_many(chk, flags=flags, want_codes=True)
     srv.drain(600000)
-    got = {c.id: (img, cod) for c, img, cod in srv.poll(4096)}
+    got = {c.id: (img, cod, c.fault_blocks) for c, img, cod in srv.poll(4096)}
     caches = {}
-    bad = 0
+    bad = checked = 0
     # The served scenes go deeper than the frame's: the plain reference renders with a stack of unknown depth.
     api.plain_config(ctx, 16, 4, 0)
     for rid, (vpi, w, h, s) in zip(ids, chk):
+        img, cod, nf = got[rid]
+        if partial[s]:  # out of core: the result is consistent with its fault count
+            bad += int((nf > 0) != bool(np.any(cod < 0)))
+            continue
+        checked += 1
         if s not in caches:
             tree, _ = scenes[s]
             caches[s] = api.GPUOctreeCacheManager(tree.info().blocks_count, tree, ctx)
             caches[s].InitialFillCache()
         out, codes, _ = api.render_plain(ctx, caches[s], w, h, vpi, flags=flags)
-        img, cod = got[rid]
         if not (np.array_equal(img, out.read_all(np.uint32)) and np.array_equal(cod, codes.read_all(np.int32))):
             bad += 1
     del caches
@@ -177,11 +205,14 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
 
     # Measured requests are delivered in the server's pinned memory and released as soon as their
     # completion is seen (the client only reads them), so the host copy to a caller buffer is not timed.
+    missing = [0]  # requests whose result lacked blocks of an out-of-core scene
+
     def collect(timeout_ms=0):
         got = srv.poll(1024, timeout_ms=timeout_ms, keep_pinned=True)
         for c, _, _ in got:
             if c.status != 0:
                 raise SystemExit(f"served request {c.id} failed ({c.status})")
+            missing[0] += int(c.fault_blocks > 0)
             srv.release(c.id)
         return len(got)
 
@@ -190,6 +221,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     reqs = make(n)
     gaps = rng.exponential(1.0 / rate, n)
     srv.reset_stats()
+    missing[0] = 0
     t0 = time.perf_counter()
     due = t0
     done = 0
@@ -215,12 +247,15 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
                  "mean_ms": round(st.mean_ms, 3), "max_ms": round(st.max_ms, 3),
                  "mrays_s": round((st.primary_rays + st.shadow_rays) / wall / 1e6, 1),
                  "rounds": int(st.rounds), "mean_views_per_round": round(st.mean_views_per_round, 2),
-                 "max_rounds_in_flight": int(st.max_rounds_in_flight), "scene_loads": int(st.scene_loads)}
+                 "max_rounds_in_flight": int(st.max_rounds_in_flight), "scene_loads": int(st.scene_loads),
+                 "block_uploads": int(st.block_uploads), "load_transactions": int(st.load_transactions),
+                 "requests_missing_blocks": missing[0]}
     log(f"served open loop: {open_loop}")
 
     # closed burst: everything queued at once
     reqs = make(burst)
     srv.reset_stats()
+    missing[0] = 0
     srv.submit_many(reqs, flags=flags, pinned=True)
     done = 0
     while done < burst:
@@ -229,11 +264,13 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     closed = {"requests": int(st.completed), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
               "mrays_s": round(st.mrays_per_s, 1), "span_ms": round(st.span_ms, 2), "rounds": int(st.rounds),
               "mean_views_per_round": round(st.mean_views_per_round, 2),
-              "max_rounds_in_flight": int(st.max_rounds_in_flight)}
+              "max_rounds_in_flight": int(st.max_rounds_in_flight), "block_uploads": int(st.block_uploads),
+              "requests_missing_blocks": missing[0]}
     log(f"served burst: {closed}")
     srv.close()
-    return {"scene_depths": [d for _, d in scenes], "pool_budget_bytes": 16 << 30, "framebuffer_bytes": 2 << 30,
-            "requests_checked_bit_exact": check, "open_loop": open_loop, "burst": closed}
+    return {"scene_depths": [d for _, d in scenes], "scene_pools": pools, "pool_budget_bytes": budget,
+            "framebuffer_bytes": 2 << 30, "requests_checked_bit_exact": checked, "open_loop": open_loop,
+            "burst": closed}
 
 
 def spline_poses(n: int, seed: int):
@@ -401,7 +438,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-burst", type=int, default=256, help="requests queued at once for the saturation run")
     ap.add_argument("--serve-depths", default="9,10,11,12",
-                    help="depths of the resident served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
+                    help="depths of the served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
+    ap.add_argument("--serve-pool-gb", type=float, default=16.0,
+                    help="block-pool budget shared by the served scenes; scenes beyond it are served out of core")
+    ap.add_argument("--serve-max-loads", type=int, default=128,
+                    help="blocks loaded per out-of-core scene between rounds (0: until its pool is full)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run a device-to-device copy of MB MiB (3 times) as the byte-counter reference")
     ap.add_argument("--no-outofcore", action="store_true")
@@ -519,11 +560,14 @@ def main():
                 scenes.append((tree, d))
                 continue
             t = time.time()
+            if d >= 13:
+                os.environ["RAYT_BUILD_LOG"] = "1"  # the builder reports its phases: minutes of work
             sspec = api.TerrainSpec(depth=d, seed=2 + k, octaves=4, boxes=64, box_emin=0.01, box_emax=0.1)
             scenes.append((api.StoredOctree.terrain(sspec, 8192), d))
             log(f"served scene depth {d} (seed {2 + k}): {scenes[-1][0].info().blocks_count} blocks, "
                 f"built in {time.time() - t:.1f}s")
-        served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst)
+        served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
+                                pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads)
         del scenes
     outofcore = None
     if not args.no_outofcore:

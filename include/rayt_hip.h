@@ -326,6 +326,8 @@ typedef struct rt_server_config {
   uint64_t framebuffer_bytes; /* device arena for request outputs */
   uint64_t pool_bytes;        /* device budget for resident scenes' block pools */
   int tile_shape, lds_levels, wgs_per_cu; /* launch shape of every round (rt_batch_config) */
+  int max_block_loads; /* blocks loaded per scene between rounds, out of core (0: until the pool is full); a
+                        * bound on the time a load holds the queue, as the reference's frame time limit */
 } rt_server_config;
 
 #define RT_OUT_IMAGE 1      /* bring the RGBA8 image to host memory */
@@ -355,6 +357,10 @@ typedef struct rt_completion {
   double done_ms;          /* result in host memory */
   const uint32_t* image;   /* RT_DELIVER_PINNED: the image in pinned memory (NULL otherwise) */
   const int32_t* codes;    /* RT_DELIVER_PINNED: the codes in pinned memory (NULL otherwise) */
+  int fault_blocks;        /* distinct tree blocks the rays found missing from the scene's pool (their pixels
+                            * show the coarser parent, as FinishTracingFrame does); 0 = full detail. Always 0
+                            * on a scene whose whole tree is resident */
+  int pad2;
 } rt_completion;
 
 typedef struct rt_server_stats {
@@ -367,6 +373,8 @@ typedef struct rt_server_stats {
   double span_ms;       /* first admission -> last completion since the last reset */
   double mrays_per_s;   /* (primary + shadow rays) / span */
   double mean_views_per_round;
+  uint64_t block_uploads;     /* blocks loaded into partial scene pools from feedback */
+  uint64_t load_transactions; /* cache transactions run between rounds for partial scenes */
 } rt_server_stats;
 
 /* Starts the server thread. */
@@ -376,6 +384,14 @@ int rt_server_destroy(rt_server* server);
 /* Registers a scene (the tree must outlive the server); tree_depth places the LDS stack window (0 = 11).
  * Returns the scene index or a negative error (RT_ERR_RANGE when its block pool exceeds pool_bytes). */
 int rt_server_add_scene(rt_server* server, rt_tree* tree, int tree_depth);
+/* The same with a block pool of max_blocks slots (<= 0: the whole tree). A pool smaller than the tree is
+ * rendered out of core: it starts with the breadth-first fill of InitialFillCache
+ * (gpu-octree-cache-manager.cpp:60-108); each round's feedback codes of the scene go through the feedback
+ * extractor (ray-tracing-feedback-extractor.cpp:38-185) and, once no round of the scene is in flight, one
+ * session / transaction (gpu-octree-cache-manager.cpp:110-212) marks the hit blocks used and loads the
+ * faulted ones, so later requests see more detail (one RenderFrame pass per request, the reference's frame
+ * after its time limit). Requests of a scene with loads due wait for its rounds in flight to finish. */
+int rt_server_add_scene_pool(rt_server* server, rt_tree* tree, int tree_depth, int max_blocks);
 /* Queues a request (thread-safe, does not wait); rejects requests that can never be admitted. */
 int rt_server_submit(rt_server* server, const rt_request* req, uint64_t* id);
 /* Queues n requests at once (all or none are accepted); ids receives their ids in order. */

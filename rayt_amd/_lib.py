@@ -139,6 +139,7 @@ SIGNATURES = {
     "rt_server_create": (I, [P, C.c_void_p, PP]),
     "rt_server_destroy": (I, [P]),
     "rt_server_add_scene": (I, [P, P, I]),
+    "rt_server_add_scene_pool": (I, [P, P, I, I]),
     "rt_server_submit": (I, [P, C.c_void_p, U64P]),
     "rt_server_submit_many": (I, [P, C.c_void_p, I, U64P]),
     "rt_server_poll": (I, [P, C.c_void_p, I, I]),
@@ -156,7 +157,8 @@ class View(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_views_per_round", I), ("rounds_in_flight", I), ("framebuffer_bytes", U64),
-                ("pool_bytes", U64), ("tile_shape", I), ("lds_levels", I), ("wgs_per_cu", I)]
+                ("pool_bytes", U64), ("tile_shape", I), ("lds_levels", I), ("wgs_per_cu", I),
+                ("max_block_loads", I)]
 
 
 RT_OUT_IMAGE = 1
@@ -173,14 +175,15 @@ class Request(C.Structure):
 
 class Completion(C.Structure):
     _fields_ = [("id", U64), ("status", I), ("round_views", I), ("submit_ms", D), ("admit_ms", D),
-                ("done_ms", D), ("image", C.c_void_p), ("codes", C.c_void_p)]
+                ("done_ms", D), ("image", C.c_void_p), ("codes", C.c_void_p), ("fault_blocks", I), ("pad2", I)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("rounds", U64), ("pixels", U64),
                 ("primary_rays", U64), ("shadow_rays", U64), ("scene_loads", U64), ("scene_evictions", U64),
                 ("max_rounds_in_flight", I), ("pad", I), ("p50_ms", D), ("p99_ms", D), ("mean_ms", D),
-                ("max_ms", D), ("span_ms", D), ("mrays_per_s", D), ("mean_views_per_round", D)]
+                ("max_ms", D), ("span_ms", D), ("mrays_per_s", D), ("mean_views_per_round", D),
+                ("block_uploads", U64), ("load_transactions", U64)]
 
 _lib = None
 

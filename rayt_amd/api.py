@@ -569,17 +569,19 @@ class RenderServer:
 
     def __init__(self, ctx: Context, max_views_per_round: int = 64, rounds_in_flight: int = 2,
                  framebuffer_bytes: int = 1 << 30, pool_bytes: int = 16 << 30, tile_shape: int = 0,
-                 lds_levels: int = 12, wgs_per_cu: int = 16):
+                 lds_levels: int = 12, wgs_per_cu: int = 24, max_block_loads: int = 0):
         cfg = L.ServerConfig(max_views_per_round, rounds_in_flight, framebuffer_bytes, pool_bytes, tile_shape,
-                             lds_levels, wgs_per_cu)
+                             lds_levels, wgs_per_cu, max_block_loads)
         h = C.c_void_p()
         call("rt_server_create", ctx.h, C.byref(cfg), C.byref(h))
         self.h, self.ctx = h, ctx
         self._trees = []
         self._live = {}  # request id -> (request struct, image array, codes array)
 
-    def add_scene(self, tree: StoredOctree, depth: int = 0) -> int:
-        idx = call("rt_server_add_scene", self.h, tree.h, depth)
+    def add_scene(self, tree: StoredOctree, depth: int = 0, max_blocks: int = 0) -> int:
+        """max_blocks > 0 and below the tree's block count: the scene is served out of core from a pool
+        of that many blocks (rt_server_add_scene_pool)."""
+        idx = call("rt_server_add_scene_pool", self.h, tree.h, depth, max_blocks)
         self._trees.append(tree)
         return idx
 

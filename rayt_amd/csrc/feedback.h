@@ -32,6 +32,8 @@ class FeedbackExtractor {
   // `codes` is the device faults_and_hits buffer (count_ ints). With
   // block_level the hit codes are already ((slot+1)<<1)|dup.
   int Process(int* codes, int nodes_in_block, bool block_level);
+  // The same over the first n (<= count) codes.
+  int ProcessN(int* codes, int n, int nodes_in_block, bool block_level);
 
   const std::vector<int>& fault_blocks() const { return faults_; }
   const std::vector<int>& hit_blocks() const { return hits_; }
@@ -69,7 +71,7 @@ class FeedbackExtractor {
       hits_.push_back((encoded >> 1) - 1);
   }
   uint32_t NextRandom();
-  int UniqueBySort(int* codes);
+  int UniqueBySort(int* codes, int n);
   int ReadLists();
 };
 

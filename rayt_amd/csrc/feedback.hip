@@ -504,17 +504,17 @@ uint32_t FeedbackExtractor::NextRandom() {
 
 // Sorted distinct codes via the reference's own sequence of steps (sort,
 // difference flags, scan, compress), used when codes leave the bitmap domain.
-int FeedbackExtractor::UniqueBySort(int* codes) {
+int FeedbackExtractor::UniqueBySort(int* codes, int n) {
   hipStream_t s = ctx_->stream;
-  int rc = SortI32(s, codes, count_, temp_);
+  int rc = SortI32(s, codes, n, temp_);
   if (rc) return rc;
-  DifferenceFlagKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, codes, index_);
-  rc = ScanI32(s, index_, count_);
+  DifferenceFlagKernel<<<dim3(Grid(n, 256)), dim3(256), 0, s>>>(n, codes, index_);
+  rc = ScanI32(s, index_, n);
   if (rc) return rc;
-  CompressKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, index_, codes, unique_);
+  CompressKernel<<<dim3(Grid(n, 256)), dim3(256), 0, s>>>(n, index_, codes, unique_);
   RT_HIP_CHECK(hipGetLastError());
   int last = 0;
-  RT_HIP_CHECK(hipMemcpyAsync(&last, index_ + (count_ - 1), 4, hipMemcpyDeviceToHost, s));
+  RT_HIP_CHECK(hipMemcpyAsync(&last, index_ + (n - 1), 4, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipStreamSynchronize(s));
   unique_count_ = last + 1;
   // fault count = number of negative distinct codes
@@ -525,8 +525,13 @@ int FeedbackExtractor::UniqueBySort(int* codes) {
 }
 
 int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level) {
+  return ProcessN(codes, count_, nodes_in_block, block_level);
+}
+
+int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool block_level) {
+  RT_REQUIRE(n >= 0 && n <= count_, RT_ERR_RANGE, "more codes than the extractor was created for");
   hipStream_t s = ctx_->stream;
-  if (count_ <= 0) {
+  if (n <= 0) {
     faults_.clear();
     hits_.clear();
     dup_hits_.clear();
@@ -535,8 +540,8 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   }
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
-  const int grid = std::min(Grid(count_, kMarkBlock * kMarkItems), max_mark_blocks_);
-  MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, count_, bs, fault_bits_, hit_bits_,
+  const int grid = std::min(Grid(n, kMarkBlock * kMarkItems), max_mark_blocks_);
+  MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, n, bs, fault_bits_, hit_bits_,
                                                    static_cast<int4*>(partials_));
   WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
                                                    static_cast<const int4*>(partials_), grid);
@@ -544,14 +549,14 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   // One round trip: the state and, speculatively, the head of the unique list
   // (usually all of it) come back together into pinned memory.
   RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));
-  prefix_len_ = std::min(count_ + 1, kPrefixInts);
+  prefix_len_ = std::min(n + 1, kPrefixInts);
   RT_HIP_CHECK(hipMemcpyAsync(host_prefix_, unique_, size_t(prefix_len_) * 4, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipStreamSynchronize(s));
   const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
   if (hs.overflow) {
     prefix_len_ = 0;  // the sort path rewrites the list
-    if (!block_level) NodeToBlockKernel<<<dim3(Grid(count_, 256)), dim3(256), 0, s>>>(count_, nodes_in_block, codes);
-    int rc = UniqueBySort(codes);
+    if (!block_level) NodeToBlockKernel<<<dim3(Grid(n, 256)), dim3(256), 0, s>>>(n, nodes_in_block, codes);
+    int rc = UniqueBySort(codes, n);
     if (rc) return rc;
   } else {
     unique_count_ = hs.unique_count;

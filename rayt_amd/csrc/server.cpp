@@ -12,6 +12,16 @@
 //      every result into the pinned mirror and the slot's completion event.
 // The first pending request that cannot be admitted blocks those behind it
 // (no overtaking), so admission is FIFO.
+//
+// A scene whose block pool is smaller than its tree is rendered out of core.
+// When a round retires, the feedback codes of each of its requests on such a
+// scene go through the feedback extractor, and the hit and fault lists are
+// merged into the scene's pending loads. Once no round of the scene is in
+// flight, one request session with one transaction marks the hit blocks used
+// and loads the faulted blocks (LRU eviction inside the scene's pool), so the
+// pool contents never change under a round that reads them. A request whose
+// scene has faults pending waits (with the requests behind it) until the
+// scene's rounds in flight have retired and the loads are done.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +37,7 @@
 
 #include "batch.h"
 #include "cache.h"
+#include "feedback.h"
 #include "rt_internal.h"
 #include "tree_handle.h"
 
@@ -81,9 +92,18 @@ struct Scene {
   int depth = 11;
   uint64_t pool_bytes = 0;
   int max_blocks = 0;
+  bool partial = false;       // the pool holds fewer blocks than the tree: out of core
   rt_cache* cache = nullptr;  // resident when non-null
   int in_flight = 0;          // admitted requests not yet retired
   uint64_t last_use = 0;
+  // Feedback of retired rounds not yet applied (slots and tree blocks as of
+  // the pool state those rounds saw: no load happens while a round is in flight).
+  std::vector<int> pend_hits, pend_dups, pend_faults;
+  void ClearPending() {
+    pend_hits.clear();
+    pend_dups.clear();
+    pend_faults.clear();
+  }
 };
 
 struct Pending {
@@ -130,6 +150,7 @@ struct rt_server {
   std::vector<Scene> scenes;
   uint64_t pool_used = 0;
   uint64_t use_clock = 0;
+  std::unique_ptr<rayt::FeedbackExtractor> fx;  // sized for the largest request of a partial scene so far
   // shared with submitters / pollers
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -154,6 +175,8 @@ struct rt_server {
   bool FormRound(Slot& s);
   void Retire(Slot& s, int status);
   int Launch(Slot& s);
+  int Feedback(Scene& sc, const Admitted& a, int* fault_blocks);
+  int ApplyPending(Scene& sc);
 };
 
 int rt_server::MakeResident(int id) {
@@ -168,6 +191,7 @@ int rt_server::MakeResident(int id) {
     if (victim < 0) return RT_ERR_NOMEM;  // every resident scene is in use: wait for rounds to retire
     rt_cache_destroy(scenes[victim].cache);
     scenes[victim].cache = nullptr;
+    scenes[victim].ClearPending();
     pool_used -= scenes[victim].pool_bytes;
     std::lock_guard<std::mutex> g(mu);
     ++st.scene_evictions;
@@ -198,9 +222,13 @@ bool rt_server::FormRound(Slot& s) {
       p = pending.front();
     }
     const uint64_t npx = uint64_t(p.req.width) * p.req.height;
+    Scene& sc = scenes[p.req.scene];
+    if (!sc.pend_faults.empty()) {  // loads due: they wait for the scene's rounds in flight
+      if (sc.in_flight > 0) break;
+      if (ApplyPending(sc) != RT_OK) fprintf(stderr, "rt_server: block loads failed: %s\n", rayt::LastError());
+    }
     uint64_t off = 0;
     if (!arena->Allocate(npx, &off)) break;  // framebuffer memory: wait for a round to retire
-    Scene& sc = scenes[p.req.scene];
     const int rc = MakeResident(p.req.scene);
     if (rc == RT_ERR_NOMEM) {  // block-pool memory: wait for a round to retire
       arena->Release(off, npx);
@@ -278,6 +306,11 @@ void rt_server::Retire(Slot& s, int status) {
   std::vector<std::pair<uint64_t, std::pair<uint64_t, uint64_t>>> hold;
   for (const Admitted& a : s.reqs) {
     const rt_request& r = a.p.req;
+    int fault_blocks = 0;
+    if (status == RT_OK && scenes[r.scene].partial) {
+      const int rc = Feedback(scenes[r.scene], a, &fault_blocks);
+      if (rc != RT_OK) fprintf(stderr, "rt_server: feedback failed: %s\n", rayt::LastError());
+    }
     const uint32_t* img = reinterpret_cast<const uint32_t*>(pinned + a.px_offset * 4);
     const int32_t* cod = reinterpret_cast<const int32_t*>(pinned + (cap + a.px_offset) * 4);
     const bool keep = status == RT_OK && r.delivery == RT_DELIVER_PINNED;
@@ -292,8 +325,12 @@ void rt_server::Retire(Slot& s, int status) {
     --scenes[r.scene].in_flight;
     done.push_back(rt_completion{a.p.id, status, int(s.reqs.size()), a.p.submit_ms, a.admit_ms, Now(),
                                  keep && (r.outputs & RT_OUT_IMAGE) ? img : nullptr,
-                                 keep && (r.outputs & RT_OUT_CODES) ? cod : nullptr});
+                                 keep && (r.outputs & RT_OUT_CODES) ? cod : nullptr, fault_blocks, 0});
   }
+  for (Scene& sc : scenes)
+    if (sc.partial && sc.in_flight == 0 && (!sc.pend_faults.empty() || !sc.pend_hits.empty()) &&
+        ApplyPending(sc) != RT_OK)
+      fprintf(stderr, "rt_server: block loads failed: %s\n", rayt::LastError());
   std::lock_guard<std::mutex> g(mu);
   for (const auto& h : hold) held[h.first] = h.second;
   for (const rt_completion& c : done) {
@@ -313,6 +350,65 @@ void rt_server::Retire(Slot& s, int status) {
   s.reqs.clear();
   s.busy = false;
   cv_done.notify_all();
+}
+
+// Feedback of one retired request on a partial scene (its codes are still in
+// the framebuffer arena): the extractor's lists join the scene's pending loads.
+int rt_server::Feedback(Scene& sc, const Admitted& a, int* fault_blocks) {
+  *fault_blocks = 0;
+  if (!sc.cache) return RT_OK;
+  const int n = int(a.px_count);
+  if (!fx || fx->count() < n) {
+    fx.reset();
+    fx.reset(new rayt::FeedbackExtractor(ctx, n));
+  }
+  int* codes = static_cast<int*>(arena_codes->ptr) + a.px_offset;
+  const int rc = fx->ProcessN(codes, n, sc.cache->mgr->nodes_in_block(), true);
+  if (rc) return rc;
+  *fault_blocks = int(fx->fault_blocks().size());
+  sc.pend_faults.insert(sc.pend_faults.end(), fx->fault_blocks().begin(), fx->fault_blocks().end());
+  sc.pend_hits.insert(sc.pend_hits.end(), fx->hit_blocks().begin(), fx->hit_blocks().end());
+  sc.pend_dups.insert(sc.pend_dups.end(), fx->duplicate_hit_blocks().begin(), fx->duplicate_hit_blocks().end());
+  return RT_OK;
+}
+
+// One request session with one transaction over the merged feedback of the
+// scene's retired rounds, as a RenderFrame pass does with one pass's lists
+// (gpu-ray-tracer.cpp:141-166): hit blocks marked used, then the faulted
+// blocks requested in descending block order until the transaction fills the
+// pool or max_block_loads are requested (the others are reported again by
+// later requests). Runs only while no round of the scene is in flight.
+int rt_server::ApplyPending(Scene& sc) {
+  auto unique = [](std::vector<int>& v, bool descending) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if (descending) std::reverse(v.begin(), v.end());
+  };
+  if (!sc.cache) {
+    sc.ClearPending();
+    return RT_OK;
+  }
+  unique(sc.pend_hits, false);
+  unique(sc.pend_dups, false);
+  unique(sc.pend_faults, true);
+  rayt::BlockCacheManager& m = *sc.cache->mgr;
+  const uint64_t before = m.cache().uploaded;
+  int rc = m.StartRequestSession();
+  if (rc == RT_OK) rc = m.StartRequestTransaction();
+  for (size_t i = 0; rc == RT_OK && i < sc.pend_hits.size(); ++i) rc = m.MarkBlockAsUsed(sc.pend_hits[i]);
+  for (size_t i = 0; rc == RT_OK && i < sc.pend_dups.size(); ++i) rc = m.MarkParentAsUsed(sc.pend_dups[i]);
+  size_t nfaults = sc.pend_faults.size();
+  if (cfg.max_block_loads > 0) nfaults = std::min(nfaults, size_t(cfg.max_block_loads));
+  for (size_t i = 0; rc == RT_OK && i < nfaults && !m.TransactionFilledCache(); ++i)
+    rc = m.RequestBlock(sc.pend_faults[i]);
+  if (rc == RT_OK) rc = m.EndRequestTransaction();
+  if (rc == RT_OK) rc = m.EndRequestSession();
+  if (rc == RT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RT_ERR_HIP;
+  sc.ClearPending();
+  std::lock_guard<std::mutex> g(mu);
+  st.block_uploads += m.cache().uploaded - before;
+  ++st.load_transactions;
+  return rc;
 }
 
 void rt_server::Loop() {
@@ -402,6 +498,7 @@ int rt_server_create(rt_ctx* ctx, const rt_server_config* cfg, rt_server** out) 
   RT_REQUIRE(cfg->framebuffer_bytes >= kBytesPerPixel && cfg->framebuffer_bytes <= (uint64_t(1) << 36), RT_ERR_RANGE,
              "framebuffer_bytes must be in [8, 64 GiB]");
   RT_REQUIRE(cfg->pool_bytes > 0, RT_ERR_RANGE, "pool_bytes must be positive");
+  RT_REQUIRE(cfg->max_block_loads >= 0, RT_ERR_RANGE, "max_block_loads must be >= 0");
   RT_TRY_BEGIN
   rt_server* s = new rt_server();
   s->ctx = rayt::CtxRetain(ctx);
@@ -467,6 +564,10 @@ int rt_server_destroy(rt_server* s) {
 }
 
 int rt_server_add_scene(rt_server* s, rt_tree* tree, int tree_depth) {
+  return rt_server_add_scene_pool(s, tree, tree_depth, 0);
+}
+
+int rt_server_add_scene_pool(rt_server* s, rt_tree* tree, int tree_depth, int max_blocks) {
   RT_REQUIRE(s && tree, RT_ERR_INVALID, "null argument");
   RT_REQUIRE(tree_depth >= 0 && tree_depth <= 22, RT_ERR_RANGE, "tree_depth must be in [0, 22]");
   rt_tree_info info;
@@ -477,9 +578,10 @@ int rt_server_add_scene(rt_server* s, rt_tree* tree, int tree_depth) {
   Scene sc;
   sc.tree = tree;
   sc.depth = tree_depth ? tree_depth : 11;
-  sc.max_blocks = int(info.blocks_count);
+  sc.max_blocks = max_blocks > 0 ? std::min(max_blocks, int(info.blocks_count)) : int(info.blocks_count);
+  sc.partial = uint32_t(sc.max_blocks) < info.blocks_count;
   // channel data + 8 far pointers + one page-table word per block slot
-  sc.pool_bytes = uint64_t(info.blocks_count) * (info.bytes_in_block_content + 8 * 4 + 4);
+  sc.pool_bytes = uint64_t(sc.max_blocks) * (info.bytes_in_block_content + 8 * 4 + 4);
   RT_REQUIRE(sc.pool_bytes <= s->cfg.pool_bytes, RT_ERR_RANGE, "scene block pool exceeds the server pool budget");
   std::lock_guard<std::mutex> g(s->mu);
   RT_REQUIRE(int(s->scenes.size()) < rayt::kMaxBatchScenes, RT_ERR_RANGE, "too many scenes");

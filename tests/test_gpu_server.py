@@ -230,3 +230,93 @@ def test_server_pinned_delivery_and_release(gpu_ctx, scenes):
             np.testing.assert_array_equal(cod, pc)
     finally:
         srv.close()
+
+
+NO_HIT = 0x7FFFFFFF
+
+
+def test_partial_scene_converges_to_the_resident_render(gpu_ctx, scenes):
+    """A scene served from a pool of 40 % of its blocks: the first result misses blocks (their pixels show
+    the coarser parent), the feedback between rounds loads them, and repeating the request converges to the
+    fully resident render (same pixels; no fault codes, the same pixels hit; hit ids are pool slots).
+    Primary rays only: a shadow ray that meets a missing block counts as unoccluded and is not fed back
+    (the pixel's code is its primary ray's), in the frame path as here."""
+    tree, _, resident, depth = scenes[1]
+    blocks = tree.info().blocks_count
+    vpi = camera_vpi(0.0, -89.0, (0.43, 0.66, 0.57), fov=30.0, aspect=1.0)
+    out, codes, _ = api.render_plain(gpu_ctx, resident, 256, 256, vpi)
+    ref_img, ref_cod = out.read_all(np.uint32), codes.read_all(np.int32)
+    srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20)
+    try:
+        s = srv.add_scene(tree, depth, max_blocks=int(blocks * 0.4))
+        faults = []
+        for _ in range(16):
+            srv.submit(vpi, 256, 256, scene=s, want_codes=True)
+            srv.drain()
+            (c, img, cod), = srv.poll(cap=4)
+            assert c.status == 0
+            faults.append(c.fault_blocks)
+            assert (c.fault_blocks > 0) == bool(np.any(cod < 0))
+            if c.fault_blocks == 0:
+                break
+        assert faults[0] > 0, "the initial fill already held every block of the view"
+        assert faults[-1] == 0, f"no convergence: faulted blocks per request {faults}"
+        np.testing.assert_array_equal(img, ref_img)
+        np.testing.assert_array_equal(cod == NO_HIT, ref_cod == NO_HIT)
+        st = srv.stats()
+        assert st.block_uploads > 0 and st.load_transactions > 0
+    finally:
+        srv.close()
+
+
+def test_partial_scene_in_a_mixed_burst(gpu_ctx, scenes):
+    """A burst over a resident scene and a partial one: the resident scene's results still equal the plain
+    render, every partial-scene result is consistent with its fault count, and loads happen between rounds."""
+    tree0, _, _, d0 = scenes[0]
+    tree1, _, _, d1 = scenes[1]
+    srv = api.RenderServer(gpu_ctx, max_views_per_round=6, rounds_in_flight=2, framebuffer_bytes=64 << 20)
+    try:
+        s0 = srv.add_scene(tree0, d0)
+        s1 = srv.add_scene(tree1, d1, max_blocks=int(tree1.info().blocks_count * 0.4))
+        reqs = [(vpi, w, h, s0 if s == 0 else s1) for vpi, w, h, s in _requests(24, 2, seed=5)]
+        ids = srv.submit_many(reqs, flags=SHADOWS, want_codes=True)
+        srv.drain()
+        got = {c.id: (c, img, cod) for c, img, cod in srv.poll(cap=64)}
+        assert sorted(got) == sorted(ids)
+        for rid, req in zip(ids, reqs):
+            c, img, cod = got[rid]
+            assert c.status == 0
+            if req[3] == s0:
+                pi, pc = _plain(gpu_ctx, scenes, (req[0], req[1], req[2], 0))
+                np.testing.assert_array_equal(img, pi)
+                np.testing.assert_array_equal(cod, pc)
+                assert c.fault_blocks == 0
+            else:
+                assert (c.fault_blocks > 0) == bool(np.any(cod < 0))
+        assert srv.stats().load_transactions > 0
+    finally:
+        srv.close()
+
+
+def test_partial_scene_load_budget(gpu_ctx, scenes):
+    """max_block_loads bounds every load between rounds; the scene still converges, in more requests."""
+    tree, _, resident, depth = scenes[1]
+    vpi = camera_vpi(0.0, -89.0, (0.43, 0.66, 0.57), fov=30.0, aspect=1.0)
+    srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20,
+                           max_block_loads=3)
+    try:
+        s = srv.add_scene(tree, depth, max_blocks=int(tree.info().blocks_count * 0.4))
+        faults = []
+        for _ in range(64):
+            srv.submit(vpi, 256, 256, scene=s)
+            srv.drain()
+            (c, _, _), = srv.poll(cap=4)
+            assert c.status == 0
+            faults.append(c.fault_blocks)
+            st = srv.stats()
+            assert st.block_uploads <= 3 * st.load_transactions
+            if c.fault_blocks == 0:
+                break
+        assert faults[0] > 3 and faults[-1] == 0, f"faulted blocks per request {faults}"
+    finally:
+        srv.close()
