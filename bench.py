@@ -311,8 +311,9 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     import numpy as np
     from rayt_amd import api
     info = tree.info()
-    # The fused single-pass frames run the plain kernel: declare this tree's depth to it.
-    api.plain_config(ctx, 16, 4, depth)
+    # The tracing passes and the fused single-pass frames keep the whole stack of this tree in LDS: the
+    # smallest window that holds it (depth - 1 levels) leaves the most LDS for resident waves.
+    api.plain_config(ctx, min(l for l in (8, 10, 12, 14, 16, 22) if l >= depth - 1), 4, depth)
     block_bytes = info.bytes_in_block_content + 36  # channels + far-pointer slots + page-table entry
     if pool_gb > 0:
         max_blocks = max(8, min(info.blocks_count, int(pool_gb * (1 << 30) / block_bytes)))

@@ -97,7 +97,7 @@ int rt_init_tracing_frame(rt_ctx* ctx, rt_buf* states, int width, int height, co
 /* RaytracingPass: GPU_renderer/kernels/raytracing_pass.cl:4-61. normals/colors may be NULL only when
  * no ray can hit. nodes_in_block == 0 writes the reference's node-level hit codes ((node+1)<<1|dup);
  * nodes_in_block > 0 writes block-level codes ((node/nodes_in_block+1)<<1|dup), i.e. NodeToBlockKernel fused.
- * flags: RT_FLAG_SHADOWS casts a shadow ray per lit hit. */
+ * flags: RT_FLAG_SHADOWS casts a shadow ray per lit hit. The traversal stack follows rt_plain_config. */
 #define RT_FLAG_SHADOWS 1
 int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int width, int height,
                        const rt_buf* node_links, const rt_buf* far_pointers, int root_node_index,
@@ -129,7 +129,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                     const rt_buf* normals, const rt_buf* colors, int nodes_in_block,
                     const float view_proj_inv[16], float lod_voxel_size, const float background[4], int flags,
                     uint64_t* stats);
-/* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context:
+/* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context, whose
+ * traversal-stack settings rt_raytracing_pass also uses (always with 256-lane workgroups):
  * lds_stack_levels (8, 10, 12, 14, 16, 22) traversal-stack levels held in LDS, wg_waves 1 (one wave per
  * workgroup, 8x8 pixels) or 4 (16x16 pixels), tree_depth of the scene (0 = unknown) to keep the deepest
  * levels in LDS. When tree_depth - 1 <= lds_stack_levels the whole stack lives in LDS (no private spill

@@ -81,13 +81,15 @@ __global__ __launch_bounds__(256) void InitTracingFrameKernel(float4* __restrict
 }
 
 // -------------------------------------------------------------- RaytracingPass
-template <int LDS_LEVELS>
+// The stack follows the context's launch shape (rt_plain_config): a window of
+// LDS_LEVELS levels from `base`, spilling to a private array (SPILL) or alone.
+template <int LDS_LEVELS, bool SPILL>
 __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ codes, float4* __restrict__ org,
                                                             const float4* __restrict__ dir, int4* __restrict__ misc,
                                                             int W, int H, PoolView pool, uint32_t root,
                                                             const uint32_t* __restrict__ normals,
                                                             const uint32_t* __restrict__ colors, int nodes_in_block,
-                                                            int flags, Lights lt) {
+                                                            int flags, Lights lt, int base) {
   __shared__ uint2 lds[LDS_LEVELS * 256];
   const uint32_t tiles_x = (W + 15) / 16, tiles = tiles_x * ((H + 15) / 16);
   int x, y;
@@ -99,16 +101,17 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
     codes[i] = kNoHit;
     return;
   }
-  StackSpill<LDS_LEVELS> spill;
-  SplitStack<LDS_LEVELS, 256> stack;
+  using Stack = SplitStack<LDS_LEVELS, 256, SPILL>;
+  StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
+  Stack stack;
   stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
+  stack.base = base;
   float4 o = org[i];
   const float4 d = dir[i];
   CastResult res;
   uint32_t fetches = 0, shadow_rays = 0;
-  CastRay<SplitStack<LDS_LEVELS, 256>, false>(pool, root, o.x, o.y, o.z, d.x, d.y, d.z, d.w, o.w, stack,
-                                              res, fetches);
+  CastRay<Stack, false>(pool, root, o.x, o.y, o.z, d.x, d.y, d.z, d.w, o.w, stack, res, fetches);
   int code;
   if (res.error_code) {
     s.y = int(PackRGBA(255, 0, 0, 255));
@@ -129,8 +132,8 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
       const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
                   nz = float(int8_t((n >> 16) & 255u)) / 127.f;
       if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
-        occluded = ShadowOccluded<SplitStack<LDS_LEVELS, 256>, false>(pool, root, o.x, o.y, o.z, d.x, d.y,
-                                                                    d.z, res, lt, stack, fetches, shadow_rays);
+        occluded = ShadowOccluded<Stack, false>(pool, root, o.x, o.y, o.z, d.x, d.y, d.z, res, lt, stack, fetches,
+                                                shadow_rays);
     }
     s.y = int(ShadeNode(normals, colors, res.hit_node, lt, occluded));
     s.z = 0;
@@ -317,12 +320,30 @@ PoolView View(const rt_buf* links, const rt_buf* far_ptrs) {
   return v;
 }
 
-template <int L>
+template <int L, bool SP>
 void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc, int W, int H,
                  const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col,
-                 int nib, int flags, const Lights& lt) {
-  RaytracingPassKernel<L><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags,
-                                                  lt);
+                 int nib, int flags, const Lights& lt, int base) {
+  RaytracingPassKernel<L, SP><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, pool, root, nrm, col, nib,
+                                                      flags, lt, base);
+}
+
+// The pass kernel for the context's stack window (levels, spill, base).
+bool DispatchPass(int levels, bool spill, dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc,
+                  int W, int H, const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col,
+                  int nib, int flags, const Lights& lt, int base) {
+#define RT_PASS(L)                                                                                           \
+  if (levels == L) {                                                                                         \
+    if (spill || L >= kStackLevels)                                                                          \
+      LaunchPassL<L, true>(g, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags, lt, base);   \
+    else                                                                                                     \
+      LaunchPassL<L, L >= kStackLevels>(g, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags,  \
+                                        lt, base);                                                           \
+    return true;                                                                                             \
+  }
+  RT_PASS(8) RT_PASS(10) RT_PASS(12) RT_PASS(14) RT_PASS(16) RT_PASS(22)
+#undef RT_PASS
+  return false;
 }
 
 template <int WV, int L, bool C, bool SP>
@@ -392,10 +413,12 @@ int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int
   RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
   char* base = static_cast<char*>(states->ptr);
   const Lights lt = rayt::MakeLights();
-  LaunchPassL<kStackLevels>(dim3(TileGrid(width, height)), ctx->stream, static_cast<int*>(faults_and_hits->ptr),
-                            reinterpret_cast<float4*>(base), reinterpret_cast<float4*>(base + 16 * P),
-                            reinterpret_cast<int4*>(base + 32 * P), width, height, View(node_links, far_pointers),
-                            uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, flags, lt);
+  const bool ok = DispatchPass(ctx->plain_lds_levels, ctx->plain_spill, dim3(TileGrid(width, height)), ctx->stream,
+                               static_cast<int*>(faults_and_hits->ptr), reinterpret_cast<float4*>(base),
+                               reinterpret_cast<float4*>(base + 16 * P), reinterpret_cast<int4*>(base + 32 * P),
+                               width, height, View(node_links, far_pointers), uint32_t(root_node_index),
+                               U32(normals), U32(colors), nodes_in_block, flags, lt, ctx->plain_lds_base);
+  RT_REQUIRE(ok, RT_ERR_STATE, "no pass kernel for this stack window");
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
 }

@@ -150,8 +150,20 @@ def test_render_counts_match_oracle(gpu_ctx):
     assert st == r["stats"]
 
 
-def test_init_pass_finish_match_oracle(gpu_ctx):
+# (LDS levels, declared depth) of the pass kernel's stack: spill from level 0 (the default), window-only,
+# spill above a window at level 3, the full LDS stack.
+@pytest.mark.parametrize("levels,depth", [(16, 0), (8, 8), (8, 12), (22, 0)])
+@pytest.mark.parametrize("flags", [0, api.L.RT_FLAG_SHADOWS])
+def test_init_pass_finish_match_oracle(gpu_ctx, levels, depth, flags):
     """InitTracingFrame, one RaytracingPass (node-level codes) and FinishTracingFrame as separate kernels."""
+    api.plain_config(gpu_ctx, levels, 4, depth)
+    try:
+        _init_pass_finish(gpu_ctx, flags)
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+
+
+def _init_pass_finish(gpu_ctx, flags):
     tree, path = terrain_tree(8, 8192, 8, 0)
     m = _gpu_cache(gpu_ctx, tree)
     t, c = _oracle_pool(path)
@@ -174,8 +186,8 @@ def test_init_pass_finish_match_oracle(gpu_ctx):
         codes = api.Buffer(gpu_ctx, P * 4)
         nrm, col = m.channel_by_name("normals"), m.channel_by_name("colors")
         api.call("rt_raytracing_pass", gpu_ctx.h, codes.h, states.h, W, H, m.channel(0).h, m.far_pointers().h,
-                 m.root_node_index(), nrm.h, col.h, 0, 0)
-        ocodes = orc.raytracing_pass(ost.reshape(-1), W, H, links, far, c.root(), normals, colors, 0, 0)
+                 m.root_node_index(), nrm.h, col.h, 0, flags)
+        ocodes = orc.raytracing_pass(ost.reshape(-1), W, H, links, far, c.root(), normals, colors, 0, flags)
         np.testing.assert_array_equal(codes.read_all(np.int32), ocodes)
         out = api.Buffer(gpu_ctx, P * 4)
         api.call("rt_finish_tracing_frame", gpu_ctx.h, out.h, states.h, W, H, api._fp(api.BACKGROUND), nrm.h, col.h)
