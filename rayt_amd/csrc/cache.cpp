@@ -2,8 +2,10 @@
 #include "cache.h"
 
 #include <algorithm>
+#include <cstring>
 #include <queue>
 #include <stdexcept>
+#include <thread>
 #include <unordered_set>
 
 #include "pool_batch.h"
@@ -88,6 +90,34 @@ uint8_t* BlockPool::StageBlock(int slot, uint32_t tree_block) {
   return p;
 }
 
+void BlockPool::StageBlockCopy(int slot, uint32_t tree_block, const uint8_t* src, bool src_stable) {
+  uint8_t* dst = StageBlock(slot, tree_block);
+  if (src_stable)
+    pending_copies_.push_back({dst, src});
+  else
+    memcpy(dst, src, content_);
+}
+
+// The payload copies of a batch: a few MB per transaction out of core, spread
+// over up to 8 threads (one thread reaches a fraction of the host's memory
+// bandwidth).
+void BlockPool::CopyPending() {
+  const size_t n = pending_copies_.size();
+  const int threads = int(std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), n / 4}));
+  auto run = [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) memcpy(pending_copies_[i].first, pending_copies_[i].second, content_);
+  };
+  if (threads <= 1) {
+    run(0, n);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(run, n * t / threads, n * (t + 1) / threads);
+    run(0, n / threads);
+    for (std::thread& th : pool) th.join();
+  }
+  pending_copies_.clear();
+}
+
 void BlockPool::StageLinkWrite(uint64_t node, uint32_t value) {
   if (link_writes_.size() >= 16ull * staging_blocks_ + 16 && Flush() != RT_OK) throw std::runtime_error(rt_last_error());
   link_writes_.push_back({node, value});
@@ -133,6 +163,7 @@ int BlockPool::Flush() {
   const uint64_t off_fwv = off;
   off += uint64_t(nfw) * 4;
   if (off > stage_bytes_) return Fail(RT_ERR_STATE, "staging overflow");
+  CopyPending();
   memcpy(host_stage_ + off_slots, staged_slots_.data(), size_t(nb) * 4);
   memcpy(host_stage_ + off_tb, staged_tree_blocks_.data(), size_t(nb) * 4);
   for (int i = 0; i < nlw; ++i) {
@@ -290,7 +321,7 @@ int BlockCache::UnloadLRUBlock() {
   return slot;
 }
 
-int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload) {
+int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable) {
   if (h.block_index >= block_to_slot_.size()) throw std::runtime_error("block index out of range");
   if (IsBlockInCache(h.block_index)) throw std::runtime_error("block already in cache");
   int slot;
@@ -314,7 +345,7 @@ int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* pay
       }
     }
   }
-  memcpy(pool_.StageBlock(slot, h.block_index), payload, pool_.content_bytes());
+  pool_.StageBlockCopy(slot, h.block_index, payload, payload_stable);
   ++uploaded;
   if (h.parent_block_index != kRootBlockParent) {
     if (!IsParentInCache(h.block_index)) throw std::runtime_error("parent block not in cache");
@@ -365,11 +396,24 @@ std::vector<int> BlockCache::LruOrder() const {
 BlockCacheManager::BlockCacheManager(rt_ctx* ctx, BlockSource* source, int max_blocks)
     : ctx_(ctx), source_(source), cache_(ctx, source->header(), max_blocks) {}
 
+// The block's header and payload: read into scratch_, or, from a source that
+// keeps records in memory, the header decoded and the payload read in place.
+const uint8_t* BlockCacheManager::FetchBlock(uint32_t index) {
+  if (const uint8_t* rec = source_->RecordPtr(index)) {
+    DecodeBlockHeader(rec, &scratch_.header);
+    fetched_in_place_ = true;
+    return rec + kBlockHeaderSize;
+  }
+  fetched_in_place_ = false;
+  return source_->LoadBlock(index, &scratch_) ? scratch_.data.data() : nullptr;
+}
+
 int BlockCacheManager::LoadAndUpload(uint32_t index) {
-  if (!source_->LoadBlock(index, &scratch_)) return Fail(RT_ERR_IO, "failed to load block");
+  const uint8_t* payload = FetchBlock(index);
+  if (!payload) return Fail(RT_ERR_IO, "failed to load block");
   const bool its_root = scratch_.header.parent_block_index == kRootBlockParent;
   const int root_in_block = scratch_.header.roots[0].pointed_child_index;
-  const int slot = cache_.UploadBlock(scratch_.header, scratch_.data.data());
+  const int slot = cache_.UploadBlock(scratch_.header, payload, fetched_in_place_);
   if (its_root) root_node_index_ = slot * nodes_in_block() + root_in_block;
   return RT_OK;
 }
@@ -383,13 +427,14 @@ int BlockCacheManager::InitialFillCache() {
     const uint32_t index = q.front();
     q.pop();
     if (cache_.IsBlockInCache(index)) continue;
-    if (!source_->LoadBlock(index, &scratch_)) return Fail(RT_ERR_IO, "failed to load block");
+    const uint8_t* payload = FetchBlock(index);
+    if (!payload) return Fail(RT_ERR_IO, "failed to load block");
     const bool its_root = scratch_.header.parent_block_index == kRootBlockParent;
     const int root_in_block = scratch_.header.roots[0].pointed_child_index;
-    const uint32_t* links = scratch_.links();
+    const uint32_t* links = reinterpret_cast<const uint32_t*>(payload);
     for (int i = 0; i < nib; ++i)
       if (LinkMask(links[i]) && LinkFault(links[i])) q.push(LinkPtr(links[i]));
-    const int slot = cache_.UploadBlock(scratch_.header, scratch_.data.data());
+    const int slot = cache_.UploadBlock(scratch_.header, payload, fetched_in_place_);
     if (its_root) root_node_index_ = slot * nib + root_in_block;
   }
   int rc = cache_.pool().Flush();

@@ -42,6 +42,9 @@ class BlockPool {
   // link writes into `slot` issued earlier in this batch are dropped (the
   // whole slot is overwritten).
   uint8_t* StageBlock(int slot, uint32_t tree_block);
+  // StageBlock + the copy of the payload. A payload that stays valid until
+  // the next Flush (src_stable) is copied there, by several threads at once.
+  void StageBlockCopy(int slot, uint32_t tree_block, const uint8_t* src, bool src_stable);
   // A 4-byte write into the node-links channel at node index `node`.
   void StageLinkWrite(uint64_t node, uint32_t value);
   void StageFarPointer(uint32_t index, uint32_t value);
@@ -66,7 +69,9 @@ class BlockPool {
   std::vector<uint32_t> staged_tree_blocks_;
   std::vector<std::pair<uint64_t, uint32_t>> link_writes_;
   std::vector<std::pair<uint32_t, uint32_t>> far_writes_;
+  std::vector<std::pair<uint8_t*, const uint8_t*>> pending_copies_;  // staging payload <- stable source
   bool in_flight_ = false;
+  void CopyPending();
   int WaitStaging();
 };
 
@@ -95,8 +100,9 @@ class BlockCache {
 
   // Uploads a block (stored link format in `payload`): picks a slot, evicting
   // the least recently used block when full, stages the bytes and the parent
-  // pointer rewrites. Returns the slot.
-  int UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload);
+  // pointer rewrites. Returns the slot. A payload that stays valid until the
+  // pool's next Flush (payload_stable) is copied to staging at the flush.
+  int UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable = false);
   std::vector<int> LruOrder() const;
 
   uint64_t uploaded = 0, evicted = 0, patches = 0, far_writes = 0;
@@ -161,6 +167,8 @@ class BlockCacheManager {
   int session_marker_ = -1, transaction_marker_ = -1;
   std::vector<int> transaction_blocks_;
   StoredOctreeBlock scratch_;
+  bool fetched_in_place_ = false;  // the last FetchBlock points into the source's memory
+  const uint8_t* FetchBlock(uint32_t index);
   int LoadAndUpload(uint32_t index);
 };
 
