@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "mark_codes.h"
 #include "rt_internal.h"
 
 namespace rayt {
@@ -32,8 +33,15 @@ class FeedbackExtractor {
   // `codes` is the device faults_and_hits buffer (count_ ints). With
   // block_level the hit codes are already ((slot+1)<<1)|dup.
   int Process(int* codes, int nodes_in_block, bool block_level);
-  // The same over the first n (<= count) codes.
-  int ProcessN(int* codes, int n, int nodes_in_block, bool block_level);
+  // The same over the first n (<= count) codes. premarked: the block-level
+  // codes were already marked into Targets() by the renderer that wrote them
+  // (plain kernel), so only the bitmap walk runs; the codes stay in memory for
+  // the sort path.
+  int ProcessN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked = false);
+  // Where a renderer marks codes for the next premarked Process: the bitmaps,
+  // with the bounds of the codes the pool can produce (tree blocks for
+  // faults, 2 * slots + 2 for hit codes).
+  MarkTargets Targets(uint32_t fault_limit, uint32_t hit_limit);
 
   const std::vector<int>& fault_blocks() const { return faults_; }
   const std::vector<int>& hit_blocks() const { return hits_; }
@@ -62,6 +70,7 @@ class FeedbackExtractor {
   int* index_ = nullptr;
   void* partials_ = nullptr;  // int4 per marking workgroup: max fault, max hit, NO_HIT seen, overflow
   int max_mark_blocks_ = 1;
+  uint32_t premark_fault_limit_ = 0, premark_hit_limit_ = 2;
   std::vector<int> host_sorted_, window_;
 
   void AddHit(int encoded) {

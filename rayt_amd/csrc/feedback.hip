@@ -14,15 +14,14 @@
 #include <vector>
 
 #include "feedback.h"
+#include "mark_codes.h"
 #include "rt_internal.h"
 
 namespace rayt {
 
 namespace {
 
-constexpr int kNoHit = 0x7fffffff;
-constexpr uint32_t kFaultDomain = 1u << 21;  // fault links carry 21-bit block indices
-constexpr uint32_t kHitDomain = 1u << 23;    // hit codes handled by the bitmap path
+constexpr int kNoHit = kMarkNoHit;
 
 // ---------------------------------------------------------------- wave scans
 __device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
@@ -246,17 +245,6 @@ constexpr int kMarkItems = 16;
 constexpr int kMarkBatch = 8;  // codes per thread loaded together
 constexpr int kMarkSlots = 4096;
 constexpr int kMarkProbe = 32;
-constexpr uint32_t kMarkEmpty = 0xffffffffu;
-constexpr uint32_t kMarkFaultKey = 0x40000000u;  // key = hit code, or fault block | this bit
-
-__device__ __forceinline__ void MarkBit(uint32_t key, uint32_t* fault_bits, uint32_t* hit_bits) {
-  if (key & kMarkFaultKey) {
-    const uint32_t f = key & ~kMarkFaultKey;
-    atomicOr(&fault_bits[f >> 5], 1u << (f & 31u));
-  } else {
-    atomicOr(&hit_bits[key >> 5], 1u << (key & 31u));
-  }
-}
 
 __global__ __launch_bounds__(kMarkBlock) void MarkCodesKernel(const int* __restrict__ codes, int n, int block_size,
                                                               uint32_t* __restrict__ fault_bits,
@@ -336,16 +324,27 @@ __global__ __launch_bounds__(kMarkBlock) void MarkCodesKernel(const int* __restr
 
 // One workgroup reduces the marking partials, walks the bitmaps in ascending
 // code order and writes the distinct codes, then clears what it read and
-// leaves the totals in the state.
+// leaves the totals in the state. premark_words > 0: the codes were marked by
+// a renderer (MarkWaveCodes): the walk covers the first premark_words.x fault
+// words and premark_words.y hit words, and reads NO_HIT and overflow from
+// bits 0 and 1 of the hit bitmap instead of the partials.
 __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
                                                           uint32_t* __restrict__ hit_bits, MarkState* __restrict__ st,
                                                           int* __restrict__ unique_out,
-                                                          const int4* __restrict__ partials, int nparts) {
+                                                          const int4* __restrict__ partials, int nparts,
+                                                          int2 premark_words) {
   __shared__ uint32_t waves[1024 / 64 + 1];
   __shared__ int red[4];
+  const bool premarked = premark_words.y > 0;
   if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
   __syncthreads();
-  {
+  if (premarked) {
+    if (threadIdx.x == 0) {
+      const uint32_t w0 = hit_bits[0];
+      red[2] = int(w0 >> kMarkNoHitKey) & 1;
+      red[3] = int(w0 >> kMarkOverflowKey) & 1;
+    }
+  } else {
     int4 acc = make_int4(-1, -1, 0, 0);
     for (int j = threadIdx.x; j < nparts; j += 1024) {
       const int4 p = partials[j];
@@ -366,12 +365,15 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
   }
   __syncthreads();
   const int max_fault = red[0], max_hit = red[1], no_hit = red[2], overflow = red[3];
-  const int wf = max_fault >= 0 ? (max_fault >> 5) + 1 : 0;
-  const int wh = max_hit >= 0 ? (max_hit >> 5) + 1 : 0;
+  const int wf = premarked ? premark_words.x : max_fault >= 0 ? (max_fault >> 5) + 1 : 0;
+  const int wh = premarked ? premark_words.y : max_hit >= 0 ? (max_hit >> 5) + 1 : 0;
+  const uint32_t flag_bits = premarked ? (1u << kMarkNoHitKey) | (1u << kMarkOverflowKey) : 0u;
   const int L = wf + wh;
   const int per = (L + 1023) / 1024;
   const int b = threadIdx.x * per, e = min(L, b + per);
-  auto word = [&](int j) -> uint32_t { return j < wf ? fault_bits[wf - 1 - j] : hit_bits[j - wf]; };
+  auto word = [&](int j) -> uint32_t {
+    return j < wf ? fault_bits[wf - 1 - j] : j == wf ? hit_bits[0] & ~flag_bits : hit_bits[j - wf];
+  };
   uint32_t cnt = 0;
   for (int j = b; j < e; ++j) cnt += __popc(word(j));
   uint32_t total;
@@ -528,7 +530,18 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   return ProcessN(codes, count_, nodes_in_block, block_level);
 }
 
-int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool block_level) {
+MarkTargets FeedbackExtractor::Targets(uint32_t fault_limit, uint32_t hit_limit) {
+  premark_fault_limit_ = std::min(fault_limit, kFaultDomain);
+  premark_hit_limit_ = std::max(2u, std::min(hit_limit, kHitDomain));
+  MarkTargets t;
+  t.fault_bits = fault_bits_;
+  t.hit_bits = hit_bits_;
+  t.fault_limit = premark_fault_limit_;
+  t.hit_limit = premark_hit_limit_;
+  return t;
+}
+
+int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked) {
   RT_REQUIRE(n >= 0 && n <= count_, RT_ERR_RANGE, "more codes than the extractor was created for");
   hipStream_t s = ctx_->stream;
   if (n <= 0) {
@@ -540,11 +553,15 @@ int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool bloc
   }
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
-  const int grid = std::min(Grid(n, kMarkBlock * kMarkItems), max_mark_blocks_);
-  MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, n, bs, fault_bits_, hit_bits_,
-                                                   static_cast<int4*>(partials_));
+  // Premarked: the renderer marked these block-level codes into Targets() as it wrote them.
+  const int grid = premarked ? 1 : std::min(Grid(n, kMarkBlock * kMarkItems), max_mark_blocks_);
+  if (!premarked)
+    MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, n, bs, fault_bits_, hit_bits_,
+                                                     static_cast<int4*>(partials_));
+  const int2 words = premarked ? make_int2(int((premark_fault_limit_ + 31) / 32), int((premark_hit_limit_ + 31) / 32))
+                               : make_int2(0, 0);
   WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
-                                                   static_cast<const int4*>(partials_), grid);
+                                                   static_cast<const int4*>(partials_), premarked ? 0 : grid, words);
   RT_HIP_CHECK(hipGetLastError());
   // One round trip: the state and, speculatively, the head of the unique list
   // (usually all of it) come back together into pinned memory.

@@ -46,6 +46,15 @@ const char* LastError();
 rt_ctx* CtxRetain(rt_ctx* ctx);
 void CtxRelease(rt_ctx* ctx);
 
+struct MarkTargets;
+// rt_render_plain, and with `mark` also the marking of its block-level codes
+// into a feedback extractor's bitmaps (FeedbackExtractor::Targets), so that
+// the extractor's Process can skip its own pass over the codes.
+int RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, int height, const rt_buf* node_links,
+                const rt_buf* far_pointers, int root_node_index, const rt_buf* normals, const rt_buf* colors,
+                int nodes_in_block, const float vpi[16], float lod_voxel_size, const float background[4], int flags,
+                uint64_t* stats, const MarkTargets* mark);
+
 }  // namespace rayt
 
 #define RT_HIP_CHECK(call)                                                                    \

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <vector>
 
+#include "mark_codes.h"
 #include "rt_internal.h"
 #include "svo_device.h"
 #include "trace_common.h"
@@ -163,11 +164,14 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 // WAVES = 4: workgroups of 256 lanes over 16x16 pixels (four 8x8 wave tiles);
 // WAVES = 1: one wave per workgroup over an 8x8 tile, so a finished wave frees
 // its LDS stack at once and more waves fit a CU.
+// mark.fault_bits != nullptr: each wave also marks its (block-level) codes
+// into the feedback extractor's bitmaps.
 template <int WAVES, int LDS_LEVELS, bool COUNT, bool SPILL>
 __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     uint32_t* __restrict__ out, int* __restrict__ codes, int W, int H, PoolView pool, uint32_t root,
     const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors, int nodes_in_block, Mat16 m,
-    float lod, Vec4 bg, int flags, Lights lt, int lds_base, unsigned long long* __restrict__ stats) {
+    float lod, Vec4 bg, int flags, Lights lt, int lds_base, unsigned long long* __restrict__ stats,
+    rayt::MarkTargets mark) {
   constexpr int BLOCK = 64 * WAVES;
   constexpr int TS = WAVES == 4 ? 16 : 8;
   __shared__ uint2 lds[LDS_LEVELS * BLOCK];
@@ -177,7 +181,9 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
   const int x = int((tile % tiles_x) * TS + (wave & 1u) * 8u + (lane & 7u));
   const int y = int((tile / tiles_x) * TS + (wave >> 1) * 8u + (lane >> 3));
   PixelCounters cnt;
-  if (x < W && y < H) {
+  int code = 0;
+  const bool inside = x < W && y < H;
+  if (inside) {
     const int64_t i = int64_t(y) * W + x;
     StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
     SplitStack<LDS_LEVELS, BLOCK, SPILL> stack;
@@ -185,12 +191,12 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     stack.lds = lds + threadIdx.x;
     stack.base = lds_base;
     uint32_t color;
-    int code;
     RenderPixel<SplitStack<LDS_LEVELS, BLOCK, SPILL>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y,
                                                              W, H, lod, bg.v, flags, lt, stack, color, code, cnt);
     out[i] = color;
     if (codes) codes[i] = code;
   }
+  if (mark.fault_bits) rayt::MarkWaveCodes(code, inside, mark);
   if (COUNT) FlushCounters(cnt, stats);
 }
 
@@ -349,9 +355,10 @@ bool DispatchPass(int levels, bool spill, dim3 g, hipStream_t s, int* codes, flo
 template <int WV, int L, bool C, bool SP>
 void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
                   uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib, const Mat16& m, float lod,
-                  const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st) {
+                  const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st,
+                  const rayt::MarkTargets& mk) {
   RenderPlainKernel<WV, L, C, SP><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod,
-                                                              bg, flags, lt, base, st);
+                                                              bg, flags, lt, base, st, mk);
 }
 
 // spill = false selects the window-only stack (the tree fits the LDS window).
@@ -359,17 +366,17 @@ template <bool C>
 bool DispatchPlain(int waves, int levels, bool spill, hipStream_t s, uint32_t* out, int* codes, int W, int H,
                    const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
                    const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, int base,
-                   unsigned long long* st) {
+                   unsigned long long* st, const rayt::MarkTargets& mk) {
   const int ts = waves == 4 ? 16 : 8;
   const dim3 g(uint32_t((W + ts - 1) / ts) * uint32_t((H + ts - 1) / ts));
 #define RT_CASE(WV, L)                                                                                      \
   if (waves == WV && levels == L) {                                                                         \
     if (spill || L >= kStackLevels)                                                                         \
       LaunchPlainL<WV, L, C, true>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, \
-                                   base, st);                                                               \
+                                   base, st, mk);                                                           \
     else                                                                                                    \
       LaunchPlainL<WV, L, C, L >= kStackLevels>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod,  \
-                                                bg, flags, lt, base, st);                                   \
+                                                bg, flags, lt, base, st, mk);                               \
     return true;                                                                                            \
   }
   RT_CASE(1, 8) RT_CASE(1, 10) RT_CASE(1, 12) RT_CASE(1, 14) RT_CASE(1, 16) RT_CASE(1, 22)
@@ -514,6 +521,17 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                     const rt_buf* far_pointers, int root_node_index, const rt_buf* normals, const rt_buf* colors,
                     int nodes_in_block, const float vpi[16], float lod_voxel_size, const float background[4],
                     int flags, uint64_t* stats) {
+  return rayt::RenderPlain(ctx, out_image, codes, width, height, node_links, far_pointers, root_node_index, normals,
+                           colors, nodes_in_block, vpi, lod_voxel_size, background, flags, stats, nullptr);
+}
+
+}  // extern "C"
+
+int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, int height,
+                      const rt_buf* node_links, const rt_buf* far_pointers, int root_node_index,
+                      const rt_buf* normals, const rt_buf* colors, int nodes_in_block, const float vpi[16],
+                      float lod_voxel_size, const float background[4], int flags, uint64_t* stats,
+                      const MarkTargets* mark) {
   int rc = CheckTraceArgs(ctx, width, height);
   if (rc) return rc;
   RT_REQUIRE(out_image && node_links && far_pointers && vpi && background, RT_ERR_INVALID, "null argument");
@@ -522,6 +540,10 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
   RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
              "root node outside pool");
   RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
+  RT_REQUIRE(!mark || (codes && nodes_in_block > 0 && !(flags & RT_FLAG_COUNT)), RT_ERR_INVALID,
+             "marking needs block-level codes and the timing variant");
+  const MarkTargets no_marks;
+  const MarkTargets& mk = mark ? *mark : no_marks;
   Mat16 m;
   memcpy(m.m, vpi, sizeof(m.m));
   Vec4 bg;
@@ -536,7 +558,7 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                         static_cast<uint32_t*>(out_image->ptr),
                         codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                         uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
-                        lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, d_stats);
+                        lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, d_stats, mk);
     hipError_t e = hipGetLastError();
     unsigned long long h[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
@@ -550,9 +572,7 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                        static_cast<uint32_t*>(out_image->ptr),
                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                        uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
-                       lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr);
+                       lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr, mk);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
 }
-
-}  // extern "C"

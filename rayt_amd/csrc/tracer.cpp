@@ -46,9 +46,9 @@ namespace {
 
 const float kBackground[4] = {0.0f, float(216 / 255.), 1.0f, 1.0f};  // gpu-ray-tracer.cpp:190
 
-int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool& done) {
+int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premarked, bool& done) {
   rayt::FeedbackExtractor& fx = *t->feedback->fx;
-  int rc = fx.Process(static_cast<int*>(t->codes->ptr), m.nodes_in_block(), true);
+  int rc = fx.ProcessN(static_cast<int*>(t->codes->ptr), fx.count(), m.nodes_in_block(), true, premarked);
   if (rc) return rc;
   rc = m.StartRequestTransaction();
   if (rc) return rc;
@@ -165,10 +165,15 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
   int iterations;
   for (iterations = 0;; ++iterations) {
     RT_HIP_CHECK(hipEventRecord(t->Event(2 * size_t(iterations)), t->ctx->stream));
+    // The first pass of a resident frame marks its codes for the feedback as it writes them (the
+    // feedback always follows the first pass).
+    const bool premarked = resident && iterations == 0;
+    const rayt::MarkTargets targets =
+        t->feedback->fx->Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
     if (resident)
-      rc = rt_render_plain(t->ctx, t->out_image, t->codes, t->width, t->height, links, far_ptrs,
-                           m.root_node_index(), normals, colors, nib, vpi, t->lod_voxel_size, kBackground, t->flags,
-                           nullptr);
+      rc = rayt::RenderPlain(t->ctx, t->out_image, t->codes, t->width, t->height, links, far_ptrs,
+                             m.root_node_index(), normals, colors, nib, vpi, t->lod_voxel_size, kBackground,
+                             t->flags, nullptr, premarked ? &targets : nullptr);
     else
       rc = rt_raytracing_pass(t->ctx, t->codes, t->states, t->width, t->height, links, far_ptrs, m.root_node_index(),
                               normals, colors, nib, t->flags);
@@ -179,7 +184,7 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
       if ((t->max_passes > 0 && iterations + 1 >= t->max_passes) || el > t->frame_time_limit) break;
     }
     bool done = false;
-    rc = FeedbackAndTransaction(t, m, done);
+    rc = FeedbackAndTransaction(t, m, premarked, done);
     if (rc) return rc;
     if (done) break;
   }
