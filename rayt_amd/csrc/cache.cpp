@@ -333,18 +333,14 @@ int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* pay
   }
   SlotInfo& info = info_[slot];
   // Enumerate child blocks (fault links); link conversion happens on the device.
-  const uint32_t* links = reinterpret_cast<const uint32_t*>(payload);
-  for (int n = 0; n < nib_; ++n) {
-    const uint32_t l = links[n];
-    if (LinkMask(l) && LinkFault(l)) {
-      const uint32_t child = LinkPtr(l);
-      if (child >= block_to_parent_slot_.size()) throw std::runtime_error("fault link outside tree");
-      if (block_to_parent_slot_[child] < 0) {
-        block_to_parent_slot_[child] = slot;
-        info.child_blocks.push_back(int32_t(child));
-      }
+  ForEachFaultLink(reinterpret_cast<const uint32_t*>(payload), nib_, [&](int, uint32_t l) {
+    const uint32_t child = LinkPtr(l);
+    if (child >= block_to_parent_slot_.size()) throw std::runtime_error("fault link outside tree");
+    if (block_to_parent_slot_[child] < 0) {
+      block_to_parent_slot_[child] = slot;
+      info.child_blocks.push_back(int32_t(child));
     }
-  }
+  });
   pool_.StageBlockCopy(slot, h.block_index, payload, payload_stable);
   ++uploaded;
   if (h.parent_block_index != kRootBlockParent) {
@@ -431,9 +427,7 @@ int BlockCacheManager::InitialFillCache() {
     if (!payload) return Fail(RT_ERR_IO, "failed to load block");
     const bool its_root = scratch_.header.parent_block_index == kRootBlockParent;
     const int root_in_block = scratch_.header.roots[0].pointed_child_index;
-    const uint32_t* links = reinterpret_cast<const uint32_t*>(payload);
-    for (int i = 0; i < nib; ++i)
-      if (LinkMask(links[i]) && LinkFault(links[i])) q.push(LinkPtr(links[i]));
+    ForEachFaultLink(reinterpret_cast<const uint32_t*>(payload), nib, [&](int, uint32_t l) { q.push(LinkPtr(l)); });
     const int slot = cache_.UploadBlock(scratch_.header, payload, fetched_in_place_);
     if (its_root) root_node_index_ = slot * nib + root_in_block;
   }

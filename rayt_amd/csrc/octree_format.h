@@ -9,6 +9,10 @@
 // reference keeps them in int, which caps pools at 2 GB).
 #pragma once
 
+#ifdef __SSE2__
+#include <emmintrin.h>
+#endif
+
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +34,33 @@ inline uint32_t PackStoredNodeLink(uint32_t children_mask, bool fault, bool dup,
 }
 inline uint32_t LinkMask(uint32_t link) { return link & 255u; }
 inline bool LinkFault(uint32_t link) { return (link >> 8) & 1u; }
+
+// Calls f(i, link) for every link among links[0, n) with children (mask != 0)
+// that are not loaded (fault bit): the child blocks of a block. A block has
+// few of them among thousands of links, so the scan tests four links per
+// SSE2 compare, (link & 0x1ff) > 0x100, and visits only the hits.
+template <class F>
+inline void ForEachFaultLink(const uint32_t* links, int n, F f) {
+  int i = 0;
+#ifdef __SSE2__
+  const __m128i low9 = _mm_set1_epi32(0x1ff), fault_only = _mm_set1_epi32(0x100);
+  for (; i + 16 <= n; i += 16) {
+    int bits = 0;
+    for (int k = 0; k < 4; ++k) {
+      const __m128i v = _mm_loadu_si128(reinterpret_cast<const __m128i*>(links + i + 4 * k));
+      const __m128i hit = _mm_cmpgt_epi32(_mm_and_si128(v, low9), fault_only);
+      bits |= _mm_movemask_ps(_mm_castsi128_ps(hit)) << (4 * k);
+    }
+    while (bits) {
+      const int k = __builtin_ctz(bits);
+      bits &= bits - 1;
+      f(i + k, links[i + k]);
+    }
+  }
+#endif
+  for (; i < n; ++i)
+    if (LinkMask(links[i]) && LinkFault(links[i])) f(i, links[i]);
+}
 inline bool LinkDup(uint32_t link) { return (link >> 9) & 1u; }
 inline bool LinkFar(uint32_t link) { return (link >> 10) & 1u; }
 inline uint32_t LinkPtr(uint32_t link) { return link >> 11; }
