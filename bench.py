@@ -37,6 +37,7 @@ def measured_traffic():
     WRITE_SIZE runs (scaled by the copy-kernel calibration recorded in the same file), or None."""
     try:
         for line in open(TRAFFIC_PROFILE):
+            line = line.strip().lstrip("-* ")
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1].strip())
     except OSError:
