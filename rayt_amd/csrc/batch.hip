@@ -65,7 +65,9 @@ struct Vec4B {
 
 // At most 80 VGPRs (six waves per SIMD): the traversal needs the latency hiding more than the
 // refill and shading code needs registers.
-template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL>
+// LOD = false: every view of the launch has lod == 0 (no LOD stop test, which
+// shadow rays never need either).
+template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void BatchKernel(
     const DevView* __restrict__ views, int nviews, const DevScene* __restrict__ scenes,
     const uint32_t* __restrict__ tile_begin, uint32_t total_tiles, uint32_t grab, uint32_t* __restrict__ counters,
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void Ba
     // pixels remain to be claimed) or no lane is tracing.
     while (true) {
       if (tracing) {
-        status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT>(tr, pool, stack, fetches);
+        status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT, LOD>(tr, pool, stack, fetches);
         tracing = status == kRunning;
       }
       if (!__any(tracing)) break;
@@ -256,12 +258,23 @@ struct LaunchArgs {
   Vec4B bg;
   Lights lt;
   unsigned long long* stats;
+  bool lod;  // some view has a nonzero LOD voxel size
 };
 
+// The timing variant comes with and without the LOD stop; the counting variant keeps it.
 template <int TW, int L, bool C, bool SP>
 void Launch(const LaunchArgs& a) {
-  BatchKernel<TW, L, C, SP><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles,
-                                                           a.grab, a.counters, a.out, a.codes, a.bg, a.lt, a.stats);
+  if constexpr (!C) {
+    if (!a.lod) {
+      BatchKernel<TW, L, C, SP, false><<<a.grid, dim3(64), 0, a.stream>>>(
+          a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles, a.grab, a.counters, a.out, a.codes, a.bg, a.lt,
+          a.stats);
+      return;
+    }
+  }
+  BatchKernel<TW, L, C, SP, true><<<a.grid, dim3(64), 0, a.stream>>>(a.views, a.nviews, a.scenes, a.tile_begin,
+                                                                     a.total_tiles, a.grab, a.counters, a.out,
+                                                                     a.codes, a.bg, a.lt, a.stats);
 }
 
 // spill = false selects the window-only stack (every scene fits the LDS window).
@@ -380,8 +393,10 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   // The staging below is read by the previous launch's copies: wait for it.
   if (b->launched) RT_HIP_CHECK(hipEventSynchronize(b->ev1));
   uint64_t tiles = 0;
+  bool any_lod = false;
   for (int i = 0; i < nviews; ++i) {
     const rt_view& v = views[i];
+    any_lod = any_lod || v.lod_voxel_size != 0.0f;
     RT_REQUIRE(v.width > 0 && v.height > 0 && int64_t(v.width) * v.height <= (int64_t(1) << 28), RT_ERR_RANGE,
                "bad view size");
     RT_REQUIRE(v.scene >= 0 && v.scene < int(b->scenes.size()), RT_ERR_RANGE, "view scene out of range");
@@ -439,6 +454,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   a.grid = dim3(uint32_t(waves));
   a.grab = uint32_t(std::clamp<uint64_t>(tiles / (waves * 16), 1, kMaxTilesPerGrab));
   a.stats = count ? b->d_stats : nullptr;
+  a.lod = any_lod;
   RT_HIP_CHECK(hipEventRecord(b->ev0, s));
   const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");

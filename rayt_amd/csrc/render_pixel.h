@@ -33,8 +33,8 @@ __device__ __forceinline__ bool ShadowOccluded(const PoolView& pool, uint32_t ro
   const float tb = cl_max(0.0f, hit.hit_t - eps);
   const float sx = ox + dx * tb, sy = oy + dy * tb, sz = oz + dz * tb;
   CastResult s;
-  CastRay<Stack, COUNT>(pool, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
-                        0.0f, 0.0f, stack, s, fetches);
+  CastRay<Stack, COUNT, false>(pool, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
+                               0.0f, 0.0f, stack, s, fetches);
   if (COUNT) ++shadow_rays;
   return s.error_code == 0 && s.fault_block == -1 && s.hit_node != -1;
 }
@@ -51,7 +51,8 @@ struct PixelCounters {
 // One pixel of a single-pass frame: InitTracingFrame, one RaytracingPass
 // (with the optional shadow ray) and FinishTracingFrame. Writes the RGBA8
 // pixel and the feedback code (block-level when nodes_in_block > 0).
-template <class Stack, bool COUNT>
+// LOD = false: lod == 0 (the primary ray has no LOD stop).
+template <class Stack, bool COUNT, bool LOD = true>
 __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root, const uint32_t* normals,
                                             const uint32_t* colors, int nodes_in_block, const float* m, int x, int y,
                                             int W, int H, float lod, const float* bg, int flags, const Lights& lt,
@@ -59,8 +60,8 @@ __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root,
   const PrimaryRay r = InitRay(m, x, y, W, H, lod);
   CastResult res;
   uint32_t fetches = 0, shadow_rays = 0;
-  CastRay<Stack, COUNT>(pool, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
-                        r.direction.z, r.direction.w, r.origin.w, stack, res, fetches);
+  CastRay<Stack, COUNT, LOD>(pool, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
+                             r.direction.z, r.direction.w, r.origin.w, stack, res, fetches);
   uint32_t color;
   int code;
   if (res.error_code) {

@@ -212,7 +212,12 @@ enum TravStatus : int {
 // the loop around the step has a single exit. The reference's loop test
 // (scale < S_MAX) is evaluated where the scale grows, at the end of a pop,
 // and ends the ray in that step.
-template <class Stack, bool COUNT>
+//
+// LOD = false drops the LOD stop test (raycast.cl:159-165) for rays whose
+// LOD coefficient and bias come from a zero LOD voxel size (lod * x is 0 or
+// NaN) and for shadow rays (coefficient and bias 0): tc_max * coef + bias is
+// then 0 or NaN, never >= scale_exp2 (> 0), so the test is always false.
+template <class Stack, bool COUNT, bool LOD = true>
 __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& stack, uint32_t& fetches) {
   if (++s.iter > kIterLimitSteps) return kIterLimit;
   if (COUNT) {
@@ -248,7 +253,9 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     child = pool.link(ofs);
     if (COUNT) ++fetches;
   }
-  const bool hit = fetch && (!(child & 255u) || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2);
+  bool stop = (child & 255u) == 0;  // a leaf
+  if constexpr (LOD) stop = stop || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2;
+  const bool hit = fetch && stop;
   const bool push = fetch && !hit;
   int status = fault ? int(kFault) : bad ? int(kBadLink) : hit ? int(kHit) : int(kRunning);
   s.child_descriptor = child;
@@ -323,7 +330,7 @@ __device__ __forceinline__ void TravResult(const Trav& s, int status, CastResult
   res.node_depth = status == kMiss ? 0 : status == kHit ? kSMax - s.scale : kSMax - s.scale - 1;
 }
 
-template <class Stack, bool COUNT>
+template <class Stack, bool COUNT, bool LOD = true>
 __device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node, float px, float py, float pz,
                                         float dx, float dy, float dz, float ray_size_coef, float ray_size_bias,
                                         Stack& stack, CastResult& res, uint32_t& fetches) {
@@ -331,7 +338,7 @@ __device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node
   TravInit(s, pool, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
   int status;
   do {
-    status = TravStep<Stack, COUNT>(s, pool, stack, fetches);
+    status = TravStep<Stack, COUNT, LOD>(s, pool, stack, fetches);
   } while (status == kRunning);
   TravResult(s, status, res);
 }

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 // its LDS stack at once and more waves fit a CU.
 // mark.fault_bits != nullptr: each wave also marks its (block-level) codes
 // into the feedback extractor's bitmaps.
-template <int WAVES, int LDS_LEVELS, bool COUNT, bool SPILL>
+template <int WAVES, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
 __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     uint32_t* __restrict__ out, int* __restrict__ codes, int W, int H, PoolView pool, uint32_t root,
     const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors, int nodes_in_block, Mat16 m,
@@ -191,8 +191,9 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     stack.lds = lds + threadIdx.x;
     stack.base = lds_base;
     uint32_t color;
-    RenderPixel<SplitStack<LDS_LEVELS, BLOCK, SPILL>, COUNT>(pool, root, normals, colors, nodes_in_block, m.m, x, y,
-                                                             W, H, lod, bg.v, flags, lt, stack, color, code, cnt);
+    RenderPixel<SplitStack<LDS_LEVELS, BLOCK, SPILL>, COUNT, LOD>(pool, root, normals, colors, nodes_in_block, m.m,
+                                                                  x, y, W, H, lod, bg.v, flags, lt, stack, color, code,
+                                                                  cnt);
     out[i] = color;
     if (codes) codes[i] = code;
   }
@@ -352,13 +353,21 @@ bool DispatchPass(int levels, bool spill, dim3 g, hipStream_t s, int* codes, flo
   return false;
 }
 
+// The timing variant comes with and without the LOD stop (lod == 0 drops it); the counting variant keeps it.
 template <int WV, int L, bool C, bool SP>
 void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
                   uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib, const Mat16& m, float lod,
                   const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st,
                   const rayt::MarkTargets& mk) {
-  RenderPlainKernel<WV, L, C, SP><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m, lod,
-                                                              bg, flags, lt, base, st, mk);
+  if constexpr (!C) {
+    if (lod == 0.0f) {
+      RenderPlainKernel<WV, L, C, SP, false><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib,
+                                                                         m, lod, bg, flags, lt, base, st, mk);
+      return;
+    }
+  }
+  RenderPlainKernel<WV, L, C, SP, true><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m,
+                                                                    lod, bg, flags, lt, base, st, mk);
 }
 
 // spill = false selects the window-only stack (the tree fits the LDS window).
