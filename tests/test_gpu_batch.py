@@ -78,6 +78,27 @@ def test_batch_spill_stack_matches_plain(gpu_ctx, scene, lds_levels, tree_depth)
         np.testing.assert_array_equal(cod[off:off + w * h], pc, err_msg=f"view {w}x{h} codes")
 
 
+def test_batch_views_with_lod(gpu_ctx, scene):
+    """Views with a LOD voxel size (some 1.5, some 0 in one launch) equal the plain render at their LOD."""
+    m = scene
+    specs = _views(10, seed=11)
+    lods = [1.5 if i % 2 else 0.0 for i in range(len(specs))]
+    flags = api.L.RT_FLAG_SHADOWS
+    b = api.BatchRenderer(m, 16)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=9)
+    views, total, offsets = b.make_views(specs, flags=flags)
+    for i, lod in enumerate(lods):
+        views[i].lod_voxel_size = lod
+    out = api.Buffer(gpu_ctx, 4 * total)
+    codes = api.Buffer(gpu_ctx, 4 * total)
+    b.render(views, len(specs), out, codes)
+    img, cod = out.read_all(np.uint32), codes.read_all(np.int32)
+    for (vpi, w, h), off, lod in zip(specs, offsets, lods):
+        po, pc, _ = api.render_plain(gpu_ctx, m, w, h, vpi, lod=lod, flags=flags)
+        np.testing.assert_array_equal(img[off:off + w * h], po.read_all(np.uint32), err_msg=f"lod {lod} image")
+        np.testing.assert_array_equal(cod[off:off + w * h], pc.read_all(np.int32), err_msg=f"lod {lod} codes")
+
+
 def test_batch_counts_add_up(gpu_ctx, scene):
     m = scene
     specs = _views(9, seed=3)
