@@ -159,7 +159,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     rng = np.random.default_rng(seed)
     budget = int(pool_gb * (1 << 30))
     srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
-                           pool_bytes=budget, tile_shape=0, lds_levels=12, wgs_per_cu=24,
+                           pool_bytes=budget, tile_shape=0, lds_levels=12, wgs_per_cu=0,
                            max_block_loads=max_block_loads)
     shares = pool_shares([t for t, _ in scenes], budget)
     partial = []

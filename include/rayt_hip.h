@@ -289,8 +289,9 @@ int rt_batch_destroy(rt_batch* batch);
 /* Registers another resident scene of the same context; returns its scene index (>= 1) or a negative error. */
 int rt_batch_add_scene(rt_batch* batch, rt_cache* cache);
 /* Launch shape: tile_shape 0 = 8x8, 1 = 16x4, 2 = 32x2 pixels per wave; lds_levels (8, 10, 12, 16) deepest
- * traversal-stack levels kept in LDS; wgs_per_cu one-wave workgroups per compute unit (1..32; the kernel is
- * built for six waves per SIMD, so more than 24 wait); tree_depth of the batch's scenes (0..22; 0 = unknown: the
+ * traversal-stack levels kept in LDS; wgs_per_cu 0 = one wave per tile, dispatched by the hardware, or 1..32
+ * persistent one-wave workgroups per compute unit (built for six waves per SIMD, so more than 24 wait);
+ * tree_depth of the batch's scenes (0..22; 0 = unknown: the
  * stack spills to a private array beyond the LDS window). With tree_depth - 1 <= lds_levels the whole stack
  * lives in LDS, under the same rule on the declared depth as rt_plain_config. */
 int rt_batch_config(rt_batch* batch, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth);

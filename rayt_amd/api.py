@@ -506,7 +506,7 @@ class BatchRenderer:
         call("rt_batch_create", cache.h, max_views, C.byref(h))
         self.h, self.cache, self.ctx = h, cache, cache.ctx
 
-    def config(self, tile_shape: int = 0, lds_levels: int = 12, wgs_per_cu: int = 16, tree_depth: int = 11):
+    def config(self, tile_shape: int = 0, lds_levels: int = 12, wgs_per_cu: int = 0, tree_depth: int = 11):
         call("rt_batch_config", self.h, tile_shape, lds_levels, wgs_per_cu, tree_depth)
 
     def add_scene(self, cache: GPUOctreeCacheManager) -> int:
@@ -569,7 +569,7 @@ class RenderServer:
 
     def __init__(self, ctx: Context, max_views_per_round: int = 64, rounds_in_flight: int = 2,
                  framebuffer_bytes: int = 1 << 30, pool_bytes: int = 16 << 30, tile_shape: int = 0,
-                 lds_levels: int = 12, wgs_per_cu: int = 24, max_block_loads: int = 0):
+                 lds_levels: int = 12, wgs_per_cu: int = 0, max_block_loads: int = 0):
         cfg = L.ServerConfig(max_views_per_round, rounds_in_flight, framebuffer_bytes, pool_bytes, tile_shape,
                              lds_levels, wgs_per_cu, max_block_loads)
         h = C.c_void_p()

@@ -40,12 +40,13 @@ def _plain(gpu_ctx, m, vpi, w, h, flags):
 
 @pytest.mark.parametrize("tile_shape", [0, 1, 2])
 @pytest.mark.parametrize("lds_levels", [8, 12, 16])
-def test_batch_matches_plain(gpu_ctx, scene, tile_shape, lds_levels):
+@pytest.mark.parametrize("wgs_per_cu", [0, 8])  # one wave per tile; persistent
+def test_batch_matches_plain(gpu_ctx, scene, tile_shape, lds_levels, wgs_per_cu):
     m = scene
     specs = _views(12, seed=tile_shape * 10 + lds_levels)
     flags = api.L.RT_FLAG_SHADOWS
     b = api.BatchRenderer(m, 64)
-    b.config(tile_shape=tile_shape, lds_levels=lds_levels, wgs_per_cu=8, tree_depth=9)
+    b.config(tile_shape=tile_shape, lds_levels=lds_levels, wgs_per_cu=wgs_per_cu, tree_depth=9)
     views, total, offsets = b.make_views(specs, flags=flags)
     out = api.Buffer(gpu_ctx, 4 * total)
     codes = api.Buffer(gpu_ctx, 4 * total)
@@ -85,7 +86,7 @@ def test_batch_views_with_lod(gpu_ctx, scene):
     lods = [1.5 if i % 2 else 0.0 for i in range(len(specs))]
     flags = api.L.RT_FLAG_SHADOWS
     b = api.BatchRenderer(m, 16)
-    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=9)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=0, tree_depth=9)
     views, total, offsets = b.make_views(specs, flags=flags)
     for i, lod in enumerate(lods):
         views[i].lod_voxel_size = lod
@@ -99,11 +100,12 @@ def test_batch_views_with_lod(gpu_ctx, scene):
         np.testing.assert_array_equal(cod[off:off + w * h], pc.read_all(np.int32), err_msg=f"lod {lod} codes")
 
 
-def test_batch_counts_add_up(gpu_ctx, scene):
+@pytest.mark.parametrize("wgs_per_cu", [0, 16])
+def test_batch_counts_add_up(gpu_ctx, scene, wgs_per_cu):
     m = scene
     specs = _views(9, seed=3)
     b = api.BatchRenderer(m, 16)
-    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=9)
+    b.config(tile_shape=0, lds_levels=12, wgs_per_cu=wgs_per_cu, tree_depth=9)
     views, total, _ = b.make_views(specs, flags=api.L.RT_FLAG_SHADOWS)
     out = api.Buffer(gpu_ctx, 4 * total)
     st = b.render(views, len(specs), out, None, count=True)
