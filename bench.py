@@ -144,7 +144,7 @@ def pool_shares(trees, budget: int):
 
 
 def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16,
-                   pool_gb: float = 16.0, max_block_loads: int = 128):
+                   pool_gb: float = 16.0, max_block_loads: int = 128, shape=(0, 12, 0)):
     """Poisson request stream over several scenes through the native continuous-batching server.
 
     scenes: list of (tree, depth), sharing a block-pool budget of `pool_gb` GiB (pool_shares: scenes that
@@ -159,7 +159,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     rng = np.random.default_rng(seed)
     budget = int(pool_gb * (1 << 30))
     srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
-                           pool_bytes=budget, tile_shape=0, lds_levels=12, wgs_per_cu=0,
+                           pool_bytes=budget, tile_shape=shape[0], lds_levels=shape[1], wgs_per_cu=shape[2],
                            max_block_loads=max_block_loads)
     shares = pool_shares([t for t, _ in scenes], budget)
     partial = []
@@ -443,6 +443,8 @@ def main():
                     help="depths of the served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
     ap.add_argument("--serve-pool-gb", type=float, default=16.0,
                     help="block-pool budget shared by the served scenes; scenes beyond it are served out of core")
+    ap.add_argument("--serve-shape", default="0,12,0", metavar="TILE,LEVELS,WGS",
+                    help="launch shape of the server's rounds (WGS 0: one wave per tile)")
     ap.add_argument("--serve-max-loads", type=int, default=128,
                     help="blocks loaded per out-of-core scene between rounds (0: until its pool is full)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -569,7 +571,8 @@ def main():
             log(f"served scene depth {d} (seed {2 + k}): {scenes[-1][0].info().blocks_count} blocks, "
                 f"built in {time.time() - t:.1f}s")
         served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
-                                pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads)
+                                pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads,
+                                shape=tuple(int(x) for x in args.serve_shape.split(",")))
         del scenes
     outofcore = None
     if not args.no_outofcore:
