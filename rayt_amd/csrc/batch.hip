@@ -295,7 +295,7 @@ __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict_
 }
 
 __global__ void ZeroCountersKernel(uint32_t* c, int n) {
-  if (int(threadIdx.x) < n) c[threadIdx.x] = 0u;
+  for (int i = int(threadIdx.x); i < n; i += int(blockDim.x)) c[i] = 0u;
 }
 
 struct LaunchArgs {
@@ -425,12 +425,12 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(DevView) * max_views);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_tile_begin), 4 * (max_views + 1));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counters), 4 * 8);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_stats), 8 * 4);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_stats), 8 * 4 * kStatSlots);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_scenes), sizeof(DevScene) * kMaxBatchScenes);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_views), sizeof(DevView) * max_views, 0);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_tile_begin), 4 * (max_views + 1), 0);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_scenes), sizeof(DevScene) * kMaxBatchScenes, 0);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stats), 8 * 4, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stats), 8 * 4 * kStatSlots, 0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e != hipSuccess) {
@@ -501,7 +501,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   RT_HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice,
                               s));
   ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8);
-  if (count) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8);
+  if (count) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8 * kStatSlots);
   LaunchArgs a;
   a.stream = s;
   a.views = b->d_views;
@@ -529,7 +529,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");
   RT_HIP_CHECK(hipGetLastError());
-  if (count) RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_stats, 8 * 4, hipMemcpyDeviceToHost, s));
+  if (count) RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_stats, 8 * 4 * kStatSlots, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipEventRecord(b->ev1, s));
   b->launched = true;
   b->counted = count;
@@ -539,7 +539,10 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
 int BatchStats(rt_batch* b, uint64_t out[4]) {
   RT_REQUIRE(b->launched && b->counted, RT_ERR_STATE, "last launch did not count");
   RT_HIP_CHECK(hipEventSynchronize(b->ev1));
-  for (int k = 0; k < 4; ++k) out[k] = b->h_stats[k];
+  for (int k = 0; k < 4; ++k) {
+    out[k] = 0;
+    for (int j = 0; j < kStatSlots; ++j) out[k] += b->h_stats[4 * j + k];
+  }
   return RT_OK;
 }
 

@@ -97,17 +97,30 @@ __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root,
   }
 }
 
-// Adds a workgroup's counters into stats[4] (one atomic per counter per workgroup).
+// Counters are kept in kStatSlots copies of {primary rays, shadow rays, node
+// fetches, hits}: each wave adds its sums into the copy of its wave index
+// (one atomic per counter per wave, spread over the copies, so a launch of
+// many one-wave workgroups does not serialise on four words), and the host
+// adds the copies.
+constexpr int kStatSlots = 64;
+
+__device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ void FlushCounters(const PixelCounters& c, unsigned long long* stats) {
-  __shared__ unsigned long long acc[4];
-  if (threadIdx.x < 4) acc[threadIdx.x] = 0;
-  __syncthreads();
-  atomicAdd(&acc[0], (unsigned long long)c.primaries);
-  atomicAdd(&acc[1], (unsigned long long)c.shadow_rays);
-  atomicAdd(&acc[2], (unsigned long long)c.fetches);
-  atomicAdd(&acc[3], (unsigned long long)c.hits);
-  __syncthreads();
-  if (threadIdx.x < 4) atomicAdd(&stats[threadIdx.x], acc[threadIdx.x]);
+  const uint32_t p = WaveSum(c.primaries), sh = WaveSum(c.shadow_rays), f = WaveSum(c.fetches),
+                 h = WaveSum(c.hits);
+  if ((threadIdx.x & 63) == 0) {
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    unsigned long long* s = stats + 4 * (wave % kStatSlots);
+    if (p) atomicAdd(&s[0], (unsigned long long)p);
+    if (sh) atomicAdd(&s[1], (unsigned long long)sh);
+    if (f) atomicAdd(&s[2], (unsigned long long)f);
+    if (h) atomicAdd(&s[3], (unsigned long long)h);
+  }
 }
 
 

@@ -556,20 +556,24 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
   if (flags & RT_FLAG_COUNT) {
     RT_REQUIRE(stats, RT_ERR_INVALID, "counting render needs a stats array");
     unsigned long long* d_stats = nullptr;
-    RT_HIP_CHECK(hipMalloc(&d_stats, 4 * sizeof(unsigned long long)));
-    hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), ctx->stream);
+    constexpr size_t kStatBytes = 4 * kStatSlots * sizeof(unsigned long long);
+    RT_HIP_CHECK(hipMalloc(&d_stats, kStatBytes));
+    hipMemsetAsync(d_stats, 0, kStatBytes, ctx->stream);
     DispatchPlain<true>(ctx->plain_waves, ctx->plain_lds_levels, ctx->plain_spill, ctx->stream,
                         static_cast<uint32_t*>(out_image->ptr),
                         codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                         uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
                         lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, d_stats, mk);
     hipError_t e = hipGetLastError();
-    unsigned long long h[4] = {0, 0, 0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    unsigned long long h[4 * kStatSlots] = {};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, kStatBytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(d_stats);
     if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
-    for (int k = 0; k < 4; ++k) stats[k] = h[k];
+    for (int k = 0; k < 4; ++k) {
+      stats[k] = 0;
+      for (int j = 0; j < kStatSlots; ++j) stats[k] += h[4 * j + k];
+    }
     return RT_OK;
   }
   DispatchPlain<false>(ctx->plain_waves, ctx->plain_lds_levels, ctx->plain_spill, ctx->stream,
