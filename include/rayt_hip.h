@@ -293,7 +293,8 @@ int rt_batch_add_scene(rt_batch* batch, rt_cache* cache);
  * persistent one-wave workgroups per compute unit (built for six waves per SIMD, so more than 24 wait);
  * tree_depth of the batch's scenes (0..22; 0 = unknown: the
  * stack spills to a private array beyond the LDS window). With tree_depth - 1 <= lds_levels the whole stack
- * lives in LDS, under the same rule on the declared depth as rt_plain_config. */
+ * lives in LDS, under the same rule on the declared depth as rt_plain_config. Default: 8x8, 12 levels, one wave
+ * per tile, depth unknown. */
 int rt_batch_config(rt_batch* batch, int tile_shape, int lds_levels, int wgs_per_cu, int tree_depth);
 /* Enqueues one batched launch on the cache's context stream. stats (may be NULL) makes it the counting variant
  * and receives {primary rays, shadow rays, node fetches, hits}. */

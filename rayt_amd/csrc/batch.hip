@@ -394,7 +394,7 @@ struct rt_batch {
   int max_views = 0;
   int tile_shape = 0;   // 0: 8x8, 1: 16x4, 2: 32x2
   int lds_levels = 12;  // 8, 10, 12 or 16
-  int wgs_per_cu = 16;  // one-wave workgroups per compute unit
+  int wgs_per_cu = 0;   // 0: one wave per tile; else persistent one-wave workgroups per compute unit
   int tree_depth = 0;   // depth assumed for scenes registered without one (0: unknown)
   std::vector<rt_cache*> scenes;
   std::vector<int> depths;  // per scene; 0 = tree_depth
