@@ -67,7 +67,7 @@ def view_mix(n: int, seed: int):
 
 
 def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: int = 7, shape=None):
-    """64-view mix in one persistent launch vs the plain kernel launched once per view.
+    """64-view mix in one batched launch vs the plain kernel launched once per view.
     shape: (tile, LDS levels, workgroups per CU) instead of the autotuner's choice."""
     import numpy as np
     from rayt_amd import api

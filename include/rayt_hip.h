@@ -36,7 +36,7 @@ typedef struct rt_tree rt_tree;     /* block source                (StoredOctree
 typedef struct rt_cache rt_cache;   /* device block pool + LRU     (GPUOctreeCacheManager) */
 typedef struct rt_feedback rt_feedback; /* feedback extraction     (RayTracingFeedbackExtractor) */
 typedef struct rt_tracer rt_tracer; /* frame renderer              (GPURayTracer) */
-typedef struct rt_batch rt_batch;   /* many views in one persistent launch (no reference counterpart) */
+typedef struct rt_batch rt_batch;   /* many views in one launch (no reference counterpart) */
 typedef struct rt_server rt_server; /* continuous-batching request loop (no reference counterpart) */
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -270,7 +270,7 @@ int rt_tracer_pass_faults(const rt_tracer* tr, int* counts, int counts_cap, int*
 int rt_tracer_timings(const rt_tracer* tr, double out_ms[2]);
 
 /* ---------------------------------------------------------------- batched
- * Many views traced by one persistent kernel (no reference counterpart: the
+ * Many views traced by one kernel launch (no reference counterpart: the
  * reference renders one view per RenderFrame). Each view's pixels and
  * feedback codes are those of rt_render_plain for that view, written at
  * pixel_offset into shared output buffers. */
@@ -317,7 +317,7 @@ int rt_batch_autotune(rt_batch* batch, const rt_view* views, int nviews, rt_buf*
  *  - its scene is resident: scenes are loaded whole into their own block
  *    pool on first use, within a device pool budget, evicting idle scenes
  *    (no request in flight) least recently used first.
- * Admitted requests form a round: one persistent batched launch
+ * Admitted requests form a round: one batched launch
  * (rt_batch_render semantics; each result equals rt_render_plain of that
  * view) on one of `rounds_in_flight` streams, followed by the copy of each
  * result to the host. Requests that arrive while rounds run are admitted into

@@ -497,7 +497,7 @@ def render_plain(ctx: Context, cache: GPUOctreeCacheManager, width: int, height:
 
 
 class BatchRenderer:
-    """Many views in one persistent launch (rt_batch_*)."""
+    """Many views in one launch (rt_batch_*): one wave per tile, or persistent waves."""
 
     TILE_SHAPES = {0: (8, 8), 1: (16, 4), 2: (32, 2)}
 
