@@ -483,15 +483,19 @@ int BlockCacheManager::RequestBlock(int block_index) {
 int BlockCacheManager::EndRequestTransaction() {
   RT_REQUIRE(transaction_, RT_ERR_STATE, "no transaction in progress");
   RT_TRY_BEGIN
+  // A transaction that loads nothing (a resident frame) enqueues no device work: no flush, no wait.
+  const bool device_work = !transaction_blocks_.empty() || !cache_.pool().empty();
   for (size_t i = 0; i < transaction_blocks_.size(); ++i) {
     if (TransactionFilledCache()) break;
     int rc = LoadAndUpload(uint32_t(transaction_blocks_[i]));
     if (rc) return rc;
   }
   transaction_blocks_.clear();
-  int rc = cache_.pool().Flush();
-  if (rc) return rc;
-  RT_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  if (device_work) {
+    int rc = cache_.pool().Flush();
+    if (rc) return rc;
+    RT_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  }
   transaction_ = false;
   cache_.ReleaseMarker(transaction_marker_);
   transaction_marker_ = -1;
