@@ -377,6 +377,74 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     return res
 
 
+def write_grid_obj(d: str, n: int) -> str:
+    """A seeded heightfield mesh: (n+1)^2 vertices, n^2 quads (2 n^2 triangles after fan splitting), vt
+    tiled 4x, one material with a colour texture and a bump map (both PNM)."""
+    import numpy as np
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(0)
+    tex = rng.integers(0, 256, size=(64, 64, 3)).astype(np.uint8)
+    open(os.path.join(d, "albedo.ppm"), "wb").write(b"P6\n64 64\n255\n" + tex.tobytes())
+    bump = rng.integers(0, 256, size=(32, 32)).astype(np.uint8)
+    open(os.path.join(d, "bump.pgm"), "wb").write(b"P5\n32 32\n255\n" + bump.tobytes())
+    open(os.path.join(d, "grid.mtl"), "w").write("newmtl g\nKa 0.05 0.05 0.05\nKd 0.9 0.9 0.9\nmap_Kd albedo.ppm\n"
+                                                  "bump bump.pgm -bm 0.3\n")
+    x = np.linspace(0.0, 1.0, n + 1)
+    X, Z = np.meshgrid(x, x, indexing="ij")
+    Y = 0.3 + 0.1 * np.sin(6 * X) * np.cos(5 * Z) + 0.05 * np.sin(17 * X + 3 * Z)
+    lines = ["mtllib grid.mtl", "usemtl g"]
+    lines += [f"v {a:.7f} {b:.7f} {c:.7f}" for a, b, c in zip(X.ravel(), Y.ravel(), Z.ravel())]
+    lines += [f"vt {4 * a:.6f} {4 * c:.6f}" for a, c in zip(X.ravel(), Z.ravel())]
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = (i * (n + 1) + j + 1).ravel()
+    b, c, e = a + n + 1, a + n + 2, a + 1
+    lines += [f"f {p}/{p} {q}/{q} {r}/{r} {s}/{s}" for p, q, r, s in zip(a, b, c, e)]
+    path = os.path.join(d, "grid.obj")
+    open(path, "w").write("\n".join(lines) + "\n")
+    return path
+
+
+def measure_obj_scene(ctx, depth: int, grid: int, steps: int, W: int, H: int):
+    """An imported OBJ scene (ImportObjScene): the heightfield mesh voxelised to `depth` on the host (timed),
+    then fully resident 1080p frames (primary + shadow rays) through the same frame path as the headline."""
+    import tempfile
+    from rayt_amd import api
+    d = os.path.join(tempfile.gettempdir(), f"rayt_bench_obj_{grid}")
+    path = write_grid_obj(d, grid)
+    t = time.perf_counter()
+    tree = api.StoredOctree.obj(path, depth, 8192)
+    import_s = time.perf_counter() - t
+    info = tree.info()
+    log(f"OBJ scene: {2 * grid * grid} triangles -> depth {depth}, {info.source_nodes} nodes, {info.blocks_count} "
+        f"blocks, imported in {import_s:.2f}s")
+    api.plain_config(ctx, min(l for l in (8, 10, 12, 14, 16, 22) if l >= depth - 1), 4, depth)
+    cache = api.GPUOctreeCacheManager(info.blocks_count, tree, ctx)
+    cache.InitialFillCache()
+    vpi = api.Camera(yaw=35.0, pitch=-25.0, position=(0.05, 0.75, 1.25), fov=60.0, aspect=W / H).view_proj_inv()
+    tr = api.GPURayTracer(cache, W, H)
+    tr.set_frame_time_limit(1e9)
+    tr.set_lod_voxel_size(0.0)
+    tr.set_flags(api.L.RT_FLAG_SHADOWS)
+    tr.set_seed(1)
+    for _ in range(3):
+        tr.RenderFrame(vpi)
+    ctx.WaitForAll()
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tr.RenderFrame(vpi)
+        rays += tr.frame_stats()[4]
+    ctx.WaitForAll()
+    el = time.perf_counter() - t0
+    res = {"triangles": 2 * grid * grid, "depth": depth, "nodes": info.source_nodes, "blocks": info.blocks_count,
+           "import_s": round(import_s, 2), "import_mnodes_s": round(info.source_nodes / import_s / 1e6, 2),
+           "host_threads": os.cpu_count(), "width": W, "height": H, "frame_ms": round(el / steps * 1e3, 3),
+           "mrays_s": round(rays / el / 1e6, 1)}
+    log(f"OBJ scene frames: {res}")
+    del tr, cache, tree
+    return res
+
+
 def measure_cpu_plain_config(ctx, threads: int):
     """The reference's CPU-runnable case: depth-8 terrain + 8 spheres, 640x480, 60 degree FOV, fixed pose,
     primary rays only, fully resident, no LOD. The oracle on all threads and on one, and the GPU kernel."""
@@ -456,6 +524,9 @@ def main():
     ap.add_argument("--ooc-check-frames", type=int, default=2, help="frames compared with the oracle")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
+    ap.add_argument("--no-obj", action="store_true")
+    ap.add_argument("--obj-depth", type=int, default=11)
+    ap.add_argument("--obj-grid", type=int, default=256, help="heightfield mesh of 2 * grid^2 triangles")
     ap.add_argument("--dump", default="", metavar="DIR", help="write the measured frame and its feedback codes as PNG")
     ap.add_argument("--plain-shape", default="4,10", metavar="WAVES,LEVELS|sweep",
                     help="plain kernel launch shape (the one the committed HBM-traffic profile measured), or "
@@ -586,6 +657,7 @@ def main():
         outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H,
                                         check_frames=args.ooc_check_frames, pool_gb=args.ooc_pool_gb)
         del otree
+    obj_scene = None if args.no_obj else measure_obj_scene(ctx, args.obj_depth, args.obj_grid, args.steps, W, H)
     cpu_plain = None if args.no_cpu_plain or args.no_cpu_baseline else \
         measure_cpu_plain_config(ctx, max(1, min(16, os.cpu_count() or 1)))
     import oracle as orc  # CPU baseline + one-frame check of the measured output
@@ -640,6 +712,7 @@ def main():
         "batched": batched,
         "served": served,
         "out_of_core": outofcore,
+        "obj_scene": obj_scene,
         "cpu_plain_config": cpu_plain,
     }
     print(json.dumps(out), flush=True)

@@ -180,6 +180,13 @@ int rt_tree_open(const char* path, rt_tree** out);
 int rt_tree_build_sphere(float cx, float cy, float cz, float radius, int level, int nodes_in_block, rt_tree** out);
 /* Seeded fBm terrain + spheres + boxes built through the same blocking builder. */
 int rt_tree_build_terrain(const rt_terrain_params* params, int nodes_in_block, rt_tree** out);
+/* ImportObjScene (scene sources/import-obj-scene.cpp:640-652) in memory: reads a Wavefront OBJ with
+ * its MTL libraries and PNM textures, voxelises it to depth max_level with channels "normals" (char4)
+ * and "colors" (uchar4) (BuilderDataSource, :529-638) and blocks it like BuildOctree; rt_tree_write
+ * then stores the file ImportObjScene writes. threads 0 = all cores. Fails with RT_ERR_IO where the
+ * reference crashes (bad numbers or faces, unknown material names, unreadable textures) and for
+ * texture formats other than PNM (.ppm/.pgm/.pnm). */
+int rt_tree_import_obj(const char* obj_path, int max_level, int nodes_in_block, int threads, rt_tree** out);
 /* StoredOctreeRawWriter output: CPU data/stored-octree-raw-writer.cpp:35-99. */
 int rt_tree_write(const rt_tree* tree, const char* path);
 int rt_tree_info_get(const rt_tree* tree, rt_tree_info* out);

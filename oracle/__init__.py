@@ -55,6 +55,11 @@ _SIG = {
                              U64P, I]),
     "orc_camera_vpi": (None, [F, F, F, FP, F, F, F, FP]),
     "orc_check_sphere_pool": (I, [U32P, U32P, U8P, U8P, I, F, F, F, F, I, I]),
+    "orc_tri_box": (I, [FP, FP, FP, FP, FP]),
+    "orc_linear_at": (F, [FP, I, I, F, F]),
+    "orc_import_obj": (C.c_int64, [C.c_char_p, I]),
+    "orc_import_obj_fetch": (None, [U8P, U8P]),
+    "orc_import_obj_error": (C.c_char_p, []),
 }
 
 BACKGROUND = np.array([0.0, np.float32(216 / 255.0), 1.0, 1.0], dtype=np.float32)
@@ -305,3 +310,30 @@ def check_sphere_pool(links, far, ch1, ch2, root, center, radius, level, faultle
     return lib().orc_check_sphere_pool(_p(links, U32P), _p(far, U32P), _p(ch1, U8P), _p(ch2, U8P), root,
                                        float(center[0]), float(center[1]), float(center[2]), float(radius), level,
                                        int(faultless))
+
+
+def tri_box(box_min, box_max, a, b, c) -> bool:
+    """TriangleBoxIntersection (util/intersections.cpp:29-59) on float32 inputs."""
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (box_min, box_max, a, b, c)]
+    return bool(lib().orc_tri_box(*[_p(x, FP) for x in arrs]))
+
+
+def linear_at(img, fx, fy) -> float:
+    """CImg 1.4.9 _linear_atXY on a one-channel float image of shape (h, w)."""
+    im = np.ascontiguousarray(img, np.float32)
+    return float(lib().orc_linear_at(_p(im, FP), im.shape[1], im.shape[0], float(fx), float(fy)))
+
+
+class ObjImportError(RuntimeError):
+    pass
+
+
+def import_obj(path: str, max_level: int):
+    """ImportObjScene's node tree: (masks uint8[n], data uint8[n, 8]) in BFS order, children in bit order."""
+    n = lib().orc_import_obj(path.encode(), max_level)
+    if n < 0:
+        raise ObjImportError(lib().orc_import_obj_error().decode())
+    masks = np.zeros(n, np.uint8)
+    data = np.zeros((n, 8), np.uint8)
+    lib().orc_import_obj_fetch(_p(masks, U8P), _p(data, U8P))
+    return masks, data

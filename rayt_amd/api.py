@@ -155,6 +155,13 @@ class StoredOctree:
         call("rt_tree_build_terrain", C.byref(p), nodes_in_block, C.byref(h))
         return StoredOctree(h)
 
+    @staticmethod
+    def obj(obj_file_name: str, max_level: int, nodes_in_block: int, threads: int = 0) -> "StoredOctree":
+        """A voxelised Wavefront OBJ scene (BuilderDataSource, scene sources/import-obj-scene.cpp:529-638)."""
+        h = C.c_void_p()
+        call("rt_tree_import_obj", obj_file_name.encode(), max_level, nodes_in_block, threads, C.byref(h))
+        return StoredOctree(h)
+
     def write(self, path: str):
         call("rt_tree_write", self.h, path.encode())
 
@@ -190,6 +197,15 @@ class StoredOctree:
             self.close()
         except Exception:
             pass
+
+
+def ImportObjScene(max_level: int, nodes_in_block: int, obj_file_name: str, out_tree_file_name: str) -> None:
+    """ImportObjScene (scene sources/import-obj-scene.cpp:640-652): OBJ file -> .tree file."""
+    t = StoredOctree.obj(obj_file_name, max_level, nodes_in_block)
+    try:
+        t.write(out_tree_file_name)
+    finally:
+        t.close()
 
 
 class GPUOctreeCacheManager:

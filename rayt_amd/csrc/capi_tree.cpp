@@ -6,6 +6,7 @@
 #include <memory>
 
 #include "builder.h"
+#include "obj_import.h"
 #include "rt_internal.h"
 #include "tree_handle.h"
 
@@ -87,6 +88,20 @@ int rt_tree_build_terrain(const rt_terrain_params* p, int nodes_in_block, rt_tre
   const auto t0 = BuildClock::now();
   rayt::NodeTree t = rayt::GenerateTerrainTree(tp);
   BuildLog("terrain nodes", t.size(), t0);
+  return FinishBuild(t, nodes_in_block, out);
+  RT_TRY_END
+}
+
+int rt_tree_import_obj(const char* obj_path, int max_level, int nodes_in_block, int threads, rt_tree** out) {
+  RT_REQUIRE(obj_path && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(max_level >= 0 && max_level <= 20, RT_ERR_RANGE, "max_level must be in [0, 20]");
+  RT_TRY_BEGIN
+  const auto t0 = BuildClock::now();
+  rayt::NodeTree t;
+  rayt::ObjImportReport rep;
+  std::string err;
+  if (!rayt::ImportObjTree(obj_path, max_level, threads, &t, &rep, &err)) return rayt::Fail(RT_ERR_IO, err);
+  BuildLog("obj nodes", t.size(), t0);
   return FinishBuild(t, nodes_in_block, out);
   RT_TRY_END
 }

@@ -9,6 +9,7 @@ import pytest
 
 import oracle as orc
 from rayt_amd import api
+from tests.obj_scenes import random_textured_scene
 from tests.scenes import POSES, camera_vpi, sphere_tree, terrain_tree
 
 pytestmark = pytest.mark.gpu
@@ -83,6 +84,35 @@ def test_render_plain_matches_oracle(gpu_ctx, lod, flags):
         r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi, lod=lod, flags=flags)
         np.testing.assert_array_equal(cod, r["codes"], err_msg=f"codes differ at pose {yaw, pitch, pos}")
         np.testing.assert_array_equal(img, r["image"], err_msg=f"image differs at pose {yaw, pitch, pos}")
+
+
+@pytest.mark.parametrize("seed,level,nib", [(0, 8, 512), (1, 10, 8192)])
+def test_render_imported_obj_matches_oracle(gpu_ctx, tmp_path, seed, level, nib):
+    """An OBJ scene (textures, bump map, vertex normals) imported by the product renders as the oracle
+    renders the same .tree file, pool words included."""
+    obj = random_textured_scene(str(tmp_path / "obj"), seed=seed, n_tris=60)
+    tree = api.StoredOctree.obj(obj, level, nib)
+    path = str(tmp_path / "scene.tree")
+    tree.write(path)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    pool = m.pool_arrays()
+    np.testing.assert_array_equal(pool["links"], c.links())
+    np.testing.assert_array_equal(pool["colors"], c.channel_bytes(2, 4))
+    W, H = 256, 192
+    hits = 0
+    for yaw, pitch, pos in POSES:
+        vpi = camera_vpi(yaw, pitch, pos)
+        out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi, flags=api.L.RT_FLAG_SHADOWS)
+        r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi,
+                             flags=api.L.RT_FLAG_SHADOWS)
+        cod = codes.read_all(np.int32).reshape(H, W)
+        np.testing.assert_array_equal(cod, r["codes"], err_msg=f"codes differ at pose {yaw, pitch, pos}")
+        np.testing.assert_array_equal(out.read_all(np.uint32).reshape(H, W), r["image"])
+        hits += int((cod != 0x7FFFFFFF).sum())
+    assert hits > 1000
+    tree.close()
 
 
 # (waves, LDS levels, declared depth): window-only stacks (depth - 1 <= levels), spill stacks with the
