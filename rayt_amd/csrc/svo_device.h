@@ -217,9 +217,13 @@ enum TravStatus : int {
 // LOD coefficient and bias come from a zero LOD voxel size (lod * x is 0 or
 // NaN) and for shadow rays (coefficient and bias 0): tc_max * coef + bias is
 // then 0 or NaN, never >= scale_exp2 (> 0), so the test is always false.
-template <class Stack, bool COUNT, bool LOD = true>
+//
+// LANE_ITER = false leaves the iteration limit (raycast.cl:104-110) to the
+// caller: CastRay starts every ray of a wave on the same loop trip, so one
+// wave-wide trip counter (a scalar register) stands for every lane's count.
+template <class Stack, bool COUNT, bool LOD = true, bool LANE_ITER = true>
 __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& stack, uint32_t& fetches) {
-  if (++s.iter > kIterLimitSteps) return kIterLimit;
+  if (LANE_ITER && ++s.iter > kIterLimitSteps) return kIterLimit;
   if (COUNT) {
     fetches += (s.pending_fetch || s.child_descriptor == 0) ? 1u : 0u;
     s.pending_fetch = 0;
@@ -246,7 +250,8 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     if (COUNT) ++fetches;
   }
   ofs += __popc(desc & (child_masks - 1u) & 255u);
-  const bool bad = far_bad || (want && !fault && (ofs >= pool.nodes || s.scale < 1));
+  // non-short-circuit forms: lane-mask ands and ors, no select through vector registers
+  const bool bad = far_bad | (want & !fault & ((ofs >= pool.nodes) | (s.scale < 1)));
   const bool fetch = want && !fault && !bad;
   uint32_t child = desc;
   if (fetch) {
@@ -266,8 +271,9 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     const float tx_center = half * s.tx_coef + tx_corner;
     const float ty_center = half * s.ty_coef + ty_corner;
     const float tz_center = half * s.tz_coef + tz_corner;
-    if (tc_max < s.h && !stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max))))
-      s.iter = kIterLimitSteps;  // deeper than a window-only stack holds: the next step reports an error
+    // deeper than a window-only stack holds: the ray ends as the iteration
+    // limit does (error 1), with the state the next step would report it from
+    if (tc_max < s.h && !stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)))) status = kIterLimit;
     s.h = tc_max;
     s.parent = ofs;
     s.scale--;
@@ -337,8 +343,13 @@ __device__ __forceinline__ void CastRay(const PoolView& pool, uint32_t root_node
   Trav s;
   TravInit(s, pool, root_node, px, py, pz, dx, dy, dz, ray_size_coef, ray_size_bias);
   int status;
+  int trips = 0;  // the same for every active lane: they all start here together
   do {
-    status = TravStep<Stack, COUNT, LOD>(s, pool, stack, fetches);
+    if (++trips > kIterLimitSteps) {
+      status = kIterLimit;
+      break;
+    }
+    status = TravStep<Stack, COUNT, LOD, false>(s, pool, stack, fetches);
   } while (status == kRunning);
   TravResult(s, status, res);
 }
