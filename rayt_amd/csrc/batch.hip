@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void Ba
             colors = sc.colors;
             root = sc.root;
             nib = sc.nodes_in_block;
-            stack.base = sc.lds_base;
+            stack.set_base(sc.lds_base);
             const PrimaryRay r = InitRay(v.m, x, y, v.width, v.height, v.lod);
             ox = r.origin.x, oy = r.origin.y, oz = r.origin.z;
             dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict_
     SplitStack<LDS_LEVELS, 64, SPILL> stack;
     stack.spill = spill.v;
     stack.lds = lds + lane;
-    stack.base = sc.lds_base;
+    stack.set_base(sc.lds_base);
     uint32_t color;
     int code;
     RenderPixel<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT, LOD>(pool, sc.root, sc.normals, sc.colors,

@@ -59,31 +59,32 @@ struct StackSpill {
 //
 // SPILL = false is the window-only stack, for trees the caller declares to
 // fit the window (depth - 1 <= LDS_LEVELS, base 0): no private array and no
-// branch per access. A level outside the window is never stored out of
-// bounds (the index is clamped) and `put` reports it, and the traversal then
+// branch per access. A level outside the window is never stored (the write
+// is masked, the read clamped into the window) and `put` reports it, and the traversal then
 // ends the ray with the iteration-limit error, so a tree deeper than declared
 // shows error pixels instead of wrong ones.
 template <int LDS_LEVELS, int BLOCK, bool SPILL = true>
 struct SplitStack {
   uint2* lds;             // base of this workgroup's [LDS_LEVELS][BLOCK] array, offset by thread
-  int base = 0;           // first level held in LDS
+  int top = kSMax - 1;    // (kSMax - 1) - first level held in LDS: a scale's LDS slot is top - scale
   uint2* spill = nullptr;  // StackSpill<LDS_LEVELS>::v of this thread (SPILL only)
+  __device__ __forceinline__ void set_base(int first_lds_level) { top = (kSMax - 1) - first_lds_level; }
   // Returns false when a window-only stack could not store the level.
   __device__ __forceinline__ bool put(int scale, uint2 v) {
-    const int level = (kSMax - 1) - scale;
     if constexpr (LDS_LEVELS >= kStackLevels) {
-      lds[level * BLOCK] = v;
+      lds[((kSMax - 1) - scale) * BLOCK] = v;
       return true;
     } else if constexpr (!SPILL) {
-      const unsigned l = unsigned(level - base);
-      lds[min(l, unsigned(LDS_LEVELS - 1)) * BLOCK] = v;
-      return l < unsigned(LDS_LEVELS);
+      const unsigned l = unsigned(top - scale);
+      const bool fits = l < unsigned(LDS_LEVELS);
+      if (fits) lds[l * BLOCK] = v;
+      return fits;
     } else {
-      const int l = level - base;
+      const int l = top - scale;
       if (unsigned(l) < unsigned(LDS_LEVELS)) {
         lds[l * BLOCK] = v;
       } else {
-        spill[level] = v;
+        spill[(kSMax - 1) - scale] = v;
         // keeps the two stores apart: merged, they become one flat store
         // through a selected pointer instead of a ds_write
         __asm__ volatile("" ::: "memory");
@@ -92,18 +93,17 @@ struct SplitStack {
     }
   }
   __device__ __forceinline__ uint2 get(int scale) const {
-    const int level = (kSMax - 1) - scale;
     if constexpr (LDS_LEVELS >= kStackLevels) {
-      return lds[level * BLOCK];
+      return lds[((kSMax - 1) - scale) * BLOCK];
     } else if constexpr (!SPILL) {
-      return lds[min(unsigned(level - base), unsigned(LDS_LEVELS - 1)) * BLOCK];
+      return lds[min(unsigned(top - scale), unsigned(LDS_LEVELS - 1)) * BLOCK];
     } else {
-      const int l = level - base;
+      const int l = top - scale;
       uint2 r;
       if (unsigned(l) < unsigned(LDS_LEVELS)) {
         r = lds[l * BLOCK];
       } else {
-        r = spill[level];
+        r = spill[(kSMax - 1) - scale];
         __asm__ volatile("" ::: "memory");
       }
       return r;
@@ -301,7 +301,9 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
       const uint32_t dx = sx ? __float_as_uint(s.posx) ^ __float_as_uint(s.posx + s.scale_exp2) : 0u;
       const uint32_t dy = sy ? __float_as_uint(s.posy) ^ __float_as_uint(s.posy + s.scale_exp2) : 0u;
       const uint32_t dz = sz ? __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2) : 0u;
-      const int scale = 31 - int(__clz(dx | dy | dz));
+      // dx | dy | dz is nonzero here: a stepped coordinate (below 2) changes
+      // when the cell size (at least 2^-23, scale >= 0) is added back to it
+      const int scale = 31 - __builtin_clz(dx | dy | dz);
       s.scale = scale;
       if (scale >= kSMax) {
         status = kMiss;  // the ray left the root cube: the loop test (raycast.cl:250, 254-255)

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
   Stack stack;
   stack.spill = spill.v;
   stack.lds = lds + threadIdx.x;
-  stack.base = base;
+  stack.set_base(base);
   float4 o = org[i];
   const float4 d = dir[i];
   CastResult res;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     SplitStack<LDS_LEVELS, BLOCK, SPILL> stack;
     stack.spill = spill.v;
     stack.lds = lds + threadIdx.x;
-    stack.base = lds_base;
+    stack.set_base(lds_base);
     uint32_t color;
     RenderPixel<SplitStack<LDS_LEVELS, BLOCK, SPILL>, COUNT, LOD>(pool, root, normals, colors, nodes_in_block, m.m,
                                                                   x, y, W, H, lod, bg.v, flags, lt, stack, color, code,
