@@ -275,6 +275,16 @@ int rt_tracer_pass_faults(const rt_tracer* tr, int* counts, int counts_cap, int*
 /* Last frame: {device ms inside tracing-pass launches (HIP events), host wall ms of RenderFrame}
  * (the PROFILE_TIMER "GPU tracing passes" / "frame" of gpu-ray-tracer.cpp:124-263). */
 int rt_tracer_timings(const rt_tracer* tr, double out_ms[2]);
+/* No reference counterpart. Pipelining of frames over a fully resident pool (default on): the call
+ * queues the frame's trace and feedback walk and returns; the frame's host half (LRU marks of its hit
+ * lists, statistics, timings) runs when the next frame's trace has been queued behind it, or when any
+ * entry point reads or changes the tracer's, its cache's or its feedback extractor's state. Results,
+ * lists, statistics and LRU order are those of the unpipelined frame; the device idles less between
+ * frames. enabled = 0 finishes a pending frame and renders every later frame to the end in the call. */
+int rt_tracer_set_pipelining(rt_tracer* tr, int enabled);
+/* No reference counterpart: device ms inside tracing-pass launches summed over every finished frame
+ * since creation, and the number of those frames (a pending frame is finished first). */
+int rt_tracer_trace_totals(rt_tracer* tr, double* trace_ms, uint64_t* frames);
 
 /* ---------------------------------------------------------------- batched
  * Many views traced by one kernel launch (no reference counterpart: the

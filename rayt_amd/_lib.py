@@ -129,6 +129,8 @@ SIGNATURES = {
     "rt_tracer_frame_stats": (I, [P, U64P]),
     "rt_tracer_pass_faults": (I, [P, IP, I, IP, I]),
     "rt_tracer_timings": (I, [P, C.POINTER(D)]),
+    "rt_tracer_set_pipelining": (I, [P, I]),
+    "rt_tracer_trace_totals": (I, [P, C.POINTER(C.c_double), U64P]),
     "rt_camera_view_proj_inv": (I, [F, F, F, FP, F, F, F, FP]),
     "rt_batch_create": (I, [P, I, PP]),
     "rt_batch_destroy": (I, [P]),

@@ -419,6 +419,15 @@ class GPURayTracer:
         call("rt_tracer_frame_stats", self.h, out)
         return list(out)
 
+    def set_pipelining(self, enabled: bool):
+        call("rt_tracer_set_pipelining", self.h, int(enabled))
+
+    def trace_totals(self) -> tuple[float, int]:
+        """(device ms in tracing passes summed over every finished frame, frames)."""
+        ms, n = C.c_double(), C.c_uint64()
+        call("rt_tracer_trace_totals", self.h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def timings(self) -> tuple[float, float]:
         """(device ms in tracing passes, host wall ms) of the last RenderFrame."""
         out = (C.c_double * 2)()

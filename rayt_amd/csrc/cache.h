@@ -178,4 +178,5 @@ struct rt_cache {
   rt_ctx* ctx = nullptr;
   rt_tree* tree = nullptr;
   std::unique_ptr<rayt::BlockCacheManager> mgr;
+  rt_tracer* pending = nullptr;  // a tracer with a pipelined frame to finish before any use
 };

@@ -4,6 +4,17 @@
 #include "cache.h"
 #include "tree_handle.h"
 
+namespace {
+// Finishes a tracer's pipelined frame before the cache's host state is read or changed.
+int Settle(const rt_cache* c) { return (c && c->pending) ? rayt::FlushPending(c->pending) : RT_OK; }
+}  // namespace
+
+#define RT_SETTLE(c)                   \
+  do {                                 \
+    const int settle_rc_ = Settle(c);  \
+    if (settle_rc_) return settle_rc_; \
+  } while (0)
+
 extern "C" {
 
 int rt_cache_create(rt_ctx* ctx, rt_tree* tree, int max_blocks_count, rt_cache** out) {
@@ -31,6 +42,7 @@ int rt_cache_create(rt_ctx* ctx, rt_tree* tree, int max_blocks_count, rt_cache**
 
 int rt_cache_destroy(rt_cache* cache) {
   if (!cache) return RT_OK;
+  (void)Settle(cache);
   rt_ctx* ctx = cache->ctx;
   delete cache;
   rayt::CtxRelease(ctx);
@@ -39,13 +51,18 @@ int rt_cache_destroy(rt_cache* cache) {
 
 int rt_cache_initial_fill(rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->InitialFillCache();
 }
 
 int rt_cache_root_node_index(const rt_cache* c) { return c ? c->mgr->root_node_index() : RT_ERR_INVALID; }
 int rt_cache_nodes_in_block(const rt_cache* c) { return c ? c->mgr->nodes_in_block() : RT_ERR_INVALID; }
 int rt_cache_max_blocks(const rt_cache* c) { return c ? c->mgr->cache().max_blocks_count() : RT_ERR_INVALID; }
-int rt_cache_loaded_blocks(const rt_cache* c) { return c ? c->mgr->cache().loaded_blocks_count() : RT_ERR_INVALID; }
+int rt_cache_loaded_blocks(const rt_cache* c) {
+  RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
+  return c->mgr->cache().loaded_blocks_count();
+}
 
 rt_buf* rt_cache_channel(rt_cache* c, int index) {
   if (!c || index < 0 || index >= c->mgr->cache().pool().channels_count()) return nullptr;
@@ -61,53 +78,63 @@ rt_buf* rt_cache_channel_by_name(rt_cache* c, const char* name) {
 rt_buf* rt_cache_far_pointers(rt_cache* c) { return c ? c->mgr->cache().pool().far_pointers() : nullptr; }
 
 int rt_cache_block_slot(const rt_cache* c, int tree_block_index) {
-  if (!c || tree_block_index < 0) return -1;
+  if (!c || tree_block_index < 0 || Settle(c)) return -1;
   return c->mgr->cache().BlockIndexInCache(uint32_t(tree_block_index));
 }
 
-int rt_cache_slot_block(const rt_cache* c, int slot) { return c ? c->mgr->cache().SlotBlock(slot) : -1; }
+int rt_cache_slot_block(const rt_cache* c, int slot) { return (c && !Settle(c)) ? c->mgr->cache().SlotBlock(slot) : -1; }
 
 int rt_cache_start_session(rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->StartRequestSession();
 }
 int rt_cache_start_transaction(rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->StartRequestTransaction();
 }
 int rt_cache_mark_block_used(rt_cache* c, int slot) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->MarkBlockAsUsed(slot);
 }
 int rt_cache_mark_parent_used(rt_cache* c, int slot) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->MarkParentAsUsed(slot);
 }
 int rt_cache_request_block(rt_cache* c, int tree_block_index) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->RequestBlock(tree_block_index);
 }
 int rt_cache_transaction_filled(const rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   RT_REQUIRE(c->mgr->transaction_in_progress(), RT_ERR_STATE, "no transaction in progress");
   return c->mgr->TransactionFilledCache() ? 1 : 0;
 }
 int rt_cache_session_filled(const rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   RT_REQUIRE(c->mgr->session_in_progress(), RT_ERR_STATE, "no session in progress");
   return c->mgr->SessionFilledCache() ? 1 : 0;
 }
 int rt_cache_end_transaction(rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->EndRequestTransaction();
 }
 int rt_cache_end_session(rt_cache* c) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   return c->mgr->EndRequestSession();
 }
 
 int rt_cache_lru_order(const rt_cache* c, int* out, int cap) {
   RT_REQUIRE(c, RT_ERR_INVALID, "null cache");
+  RT_SETTLE(c);
   std::vector<int> v = c->mgr->cache().LruOrder();
   for (int i = 0; i < cap && i < int(v.size()); ++i) out[i] = v[i];
   return int(v.size());
@@ -115,6 +142,7 @@ int rt_cache_lru_order(const rt_cache* c, int* out, int cap) {
 
 int rt_cache_counters(const rt_cache* c, uint64_t out[4]) {
   RT_REQUIRE(c && out, RT_ERR_INVALID, "null argument");
+  RT_SETTLE(c);
   const auto& k = c->mgr->cache();
   out[0] = k.uploaded;
   out[1] = k.evicted;

@@ -38,6 +38,12 @@ class FeedbackExtractor {
   // (plain kernel), so only the bitmap walk runs; the codes stay in memory for
   // the sort path.
   int ProcessN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked = false);
+  // ProcessN in two halves, for a caller that overlaps the host half with
+  // later device work: EnqueueN issues the marking, the walk and the copy of
+  // its result; CompleteN waits for that copy and reads the lists. The codes
+  // must not change in between (the sort path may read them).
+  int EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked);
+  int CompleteN();
   // Where a renderer marks codes for the next premarked Process: the bitmaps,
   // with the bounds of the codes the pool can produce (tree blocks for
   // faults, 2 * slots + 2 for hit codes).
@@ -56,6 +62,11 @@ class FeedbackExtractor {
   int expected_faults_ = 1 << 16;
   uint64_t rng_ = 0;
   int unique_count_ = 0, fault_count_ = 0;
+  // the walk issued by EnqueueN that CompleteN finishes
+  hipEvent_t copied_ = nullptr;
+  int* q_codes_ = nullptr;
+  int q_n_ = 0, q_nib_ = 0;
+  bool q_block_level_ = false, q_empty_ = true;
   std::vector<int> faults_, hits_, dup_hits_;
   uint32_t* fault_bits_ = nullptr;
   uint32_t* hit_bits_ = nullptr;
@@ -88,4 +99,5 @@ class FeedbackExtractor {
 
 struct rt_feedback {
   std::unique_ptr<rayt::FeedbackExtractor> fx;
+  rt_tracer* pending = nullptr;  // the owning tracer while it has a pipelined frame to finish
 };

@@ -328,11 +328,16 @@ __global__ __launch_bounds__(kMarkBlock) void MarkCodesKernel(const int* __restr
 // a renderer (MarkWaveCodes): the walk covers the first premark_words.x fault
 // words and premark_words.y hit words, and reads NO_HIT and overflow from
 // bits 0 and 1 of the hit bitmap instead of the partials.
+//
+// The state and the first prefix_len distinct codes also go straight to
+// pinned host memory (host_state, host_prefix): the host reads them after the
+// kernel, with no copy engine between this kernel and the next one.
 __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
                                                           uint32_t* __restrict__ hit_bits, MarkState* __restrict__ st,
                                                           int* __restrict__ unique_out,
                                                           const int4* __restrict__ partials, int nparts,
-                                                          int2 premark_words) {
+                                                          int2 premark_words, MarkState* __restrict__ host_state,
+                                                          int* __restrict__ host_prefix, int prefix_len) {
   __shared__ uint32_t waves[1024 / 64 + 1];
   __shared__ int red[4];
   const bool premarked = premark_words.y > 0;
@@ -405,15 +410,36 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
     }
   }
   __syncthreads();
+  const int u = int(total) + (no_hit ? 1 : 0);
+  MarkState fin;
+  fin.unique_count = u;
+  fin.fault_count = int(faults_total);
+  fin.max_fault = max_fault;
+  fin.max_hit = max_hit;
+  fin.no_hit = no_hit;
+  fin.overflow = overflow;
   if (threadIdx.x == 0) {
-    int u = int(total);
-    if (no_hit) unique_out[u++] = kNoHit;
-    st->unique_count = u;
-    st->fault_count = int(faults_total);
-    st->max_fault = max_fault;
-    st->max_hit = max_hit;
-    st->no_hit = no_hit;
-    st->overflow = overflow;
+    if (no_hit) unique_out[u - 1] = kNoHit;
+    st->unique_count = fin.unique_count;
+    st->fault_count = fin.fault_count;
+    st->max_fault = fin.max_fault;
+    st->max_hit = fin.max_hit;
+    st->no_hit = fin.no_hit;
+    st->overflow = fin.overflow;
+  }
+  if (host_state) {
+    __syncthreads();  // the NO_HIT entry is in place
+    const int m = min(u, prefix_len);
+    for (int i = threadIdx.x; i < m; i += 1024) host_prefix[i] = unique_out[i];
+    if (threadIdx.x == 0) {
+      host_state->unique_count = fin.unique_count;
+      host_state->fault_count = fin.fault_count;
+      host_state->max_fault = fin.max_fault;
+      host_state->max_hit = fin.max_hit;
+      host_state->no_hit = fin.no_hit;
+      host_state->overflow = fin.overflow;
+    }
+    __threadfence_system();
   }
 }
 
@@ -479,6 +505,8 @@ FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(CtxRetain(ct
   e = hipHostMalloc(&host_prefix_, size_t(kPrefixInts) * 4, hipHostMallocDefault);
   if (e != hipSuccess) throw std::runtime_error("hipHostMalloc");
   host_sorted_.resize(size_t(std::max(count, 1)) + 1);
+  e = hipEventCreateWithFlags(&copied_, hipEventDisableTiming);
+  if (e != hipSuccess) throw std::runtime_error("hipEventCreateWithFlags");
   (void)hipStreamSynchronize(ctx_->stream);
 }
 
@@ -493,6 +521,7 @@ FeedbackExtractor::~FeedbackExtractor() {
   (void)hipFree(partials_);
   (void)hipHostFree(host_state_);
   (void)hipHostFree(host_prefix_);
+  if (copied_) (void)hipEventDestroy(copied_);
   CtxRelease(ctx_);
 }
 
@@ -542,15 +571,19 @@ MarkTargets FeedbackExtractor::Targets(uint32_t fault_limit, uint32_t hit_limit)
 }
 
 int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked) {
+  const int rc = EnqueueN(codes, n, nodes_in_block, block_level, premarked);
+  return rc ? rc : CompleteN();
+}
+
+int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked) {
   RT_REQUIRE(n >= 0 && n <= count_, RT_ERR_RANGE, "more codes than the extractor was created for");
   hipStream_t s = ctx_->stream;
-  if (n <= 0) {
-    faults_.clear();
-    hits_.clear();
-    dup_hits_.clear();
-    unique_count_ = 0;
-    return RT_OK;
-  }
+  q_codes_ = codes;
+  q_n_ = n;
+  q_nib_ = nodes_in_block;
+  q_block_level_ = block_level;
+  q_empty_ = n <= 0;
+  if (q_empty_) return RT_OK;
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
   // Premarked: the renderer marked these block-level codes into Targets() as it wrote them.
@@ -560,20 +593,33 @@ int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool bloc
                                                      static_cast<int4*>(partials_));
   const int2 words = premarked ? make_int2(int((premark_fault_limit_ + 31) / 32), int((premark_hit_limit_ + 31) / 32))
                                : make_int2(0, 0);
-  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
-                                                   static_cast<const int4*>(partials_), premarked ? 0 : grid, words);
-  RT_HIP_CHECK(hipGetLastError());
-  // One round trip: the state and, speculatively, the head of the unique list
-  // (usually all of it) come back together into pinned memory.
-  RT_HIP_CHECK(hipMemcpyAsync(host_state_, state_, sizeof(MarkState), hipMemcpyDeviceToHost, s));
+  // The walk writes the state and, speculatively, the head of the unique
+  // list (usually all of it) into pinned host memory itself.
   prefix_len_ = std::min(n + 1, kPrefixInts);
-  RT_HIP_CHECK(hipMemcpyAsync(host_prefix_, unique_, size_t(prefix_len_) * 4, hipMemcpyDeviceToHost, s));
-  RT_HIP_CHECK(hipStreamSynchronize(s));
+  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+                                                   static_cast<const int4*>(partials_), premarked ? 0 : grid, words,
+                                                   static_cast<MarkState*>(host_state_),
+                                                   static_cast<int*>(host_prefix_), prefix_len_);
+  RT_HIP_CHECK(hipGetLastError());
+  RT_HIP_CHECK(hipEventRecord(copied_, s));
+  return RT_OK;
+}
+
+int FeedbackExtractor::CompleteN() {
+  if (q_empty_) {
+    faults_.clear();
+    hits_.clear();
+    dup_hits_.clear();
+    unique_count_ = 0;
+    return RT_OK;
+  }
+  hipStream_t s = ctx_->stream;
+  RT_HIP_CHECK(hipEventSynchronize(copied_));
   const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
   if (hs.overflow) {
     prefix_len_ = 0;  // the sort path rewrites the list
-    if (!block_level) NodeToBlockKernel<<<dim3(Grid(n, 256)), dim3(256), 0, s>>>(n, nodes_in_block, codes);
-    int rc = UniqueBySort(codes, n);
+    if (!q_block_level_) NodeToBlockKernel<<<dim3(Grid(q_n_, 256)), dim3(256), 0, s>>>(q_n_, q_nib_, q_codes_);
+    int rc = UniqueBySort(q_codes_, q_n_);
     if (rc) return rc;
   } else {
     unique_count_ = hs.unique_count;
@@ -653,6 +699,11 @@ int FeedbackExtractor::ReadLists() {
 }  // namespace rayt
 
 // -------------------------------------------------------------------- C ABI
+namespace {
+// Finishes the owning tracer's pipelined frame before the extractor's state is read or changed.
+int Settle(const rt_feedback* fb) { return (fb && fb->pending) ? rayt::FlushPending(fb->pending) : RT_OK; }
+}  // namespace
+
 extern "C" {
 
 int rt_node_to_block(rt_ctx* ctx, int n, int block_size, rt_buf* data) {
@@ -718,12 +769,14 @@ int rt_feedback_create(rt_ctx* ctx, int count, rt_feedback** out) {
 }
 
 int rt_feedback_destroy(rt_feedback* fb) {
+  (void)Settle(fb);
   delete fb;
   return RT_OK;
 }
 
 int rt_feedback_set_hits_to_extract(rt_feedback* fb, int value) {
   RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  if (const int rc = Settle(fb)) return rc;
   RT_REQUIRE(value >= 0, RT_ERR_RANGE, "negative hits_to_extract");
   fb->fx->set_hits_to_extract(value);
   return RT_OK;
@@ -731,12 +784,14 @@ int rt_feedback_set_hits_to_extract(rt_feedback* fb, int value) {
 
 int rt_feedback_set_seed(rt_feedback* fb, uint64_t seed) {
   RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  if (const int rc = Settle(fb)) return rc;
   fb->fx->set_seed(seed);
   return RT_OK;
 }
 
 int rt_feedback_process(rt_feedback* fb, rt_buf* codes, int nodes_in_block, int block_level) {
   RT_REQUIRE(fb && codes, RT_ERR_INVALID, "null argument");
+  if (const int rc = Settle(fb)) return rc;
   RT_REQUIRE(uint64_t(fb->fx->count()) * 4 <= codes->size, RT_ERR_RANGE, "codes buffer smaller than count");
   RT_REQUIRE(block_level || nodes_in_block > 0, RT_ERR_INVALID, "nodes_in_block must be positive");
   RT_TRY_BEGIN
@@ -746,6 +801,7 @@ int rt_feedback_process(rt_feedback* fb, rt_buf* codes, int nodes_in_block, int 
 
 int rt_feedback_list(const rt_feedback* fb, int which, int* out, int cap) {
   RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  if (const int rc = Settle(fb)) return rc;
   const std::vector<int>* v = which == 0 ? &fb->fx->fault_blocks() : which == 1 ? &fb->fx->hit_blocks()
                                                                                : &fb->fx->duplicate_hit_blocks();
   RT_REQUIRE(which >= 0 && which <= 2, RT_ERR_RANGE, "which must be 0, 1 or 2");
@@ -755,6 +811,7 @@ int rt_feedback_list(const rt_feedback* fb, int which, int* out, int cap) {
 
 int rt_feedback_stats(const rt_feedback* fb, int* unique_count, int* expected_faults) {
   RT_REQUIRE(fb, RT_ERR_INVALID, "null feedback");
+  if (const int rc = Settle(fb)) return rc;
   if (unique_count) *unique_count = fb->fx->unique_count();
   if (expected_faults) *expected_faults = fb->fx->expected_faults();
   return RT_OK;

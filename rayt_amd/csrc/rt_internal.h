@@ -35,6 +35,8 @@ struct rt_evt {
   std::atomic<int> refs{1};
 };
 
+struct rt_tracer;
+
 namespace rayt {
 
 void SetError(const std::string& msg);
@@ -54,6 +56,13 @@ int RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, int he
                 const rt_buf* far_pointers, int root_node_index, const rt_buf* normals, const rt_buf* colors,
                 int nodes_in_block, const float vpi[16], float lod_voxel_size, const float background[4], int flags,
                 uint64_t* stats, const MarkTargets* mark);
+
+// A tracer whose last frame was pipelined (rt_tracer_render_frame on a fully
+// resident pool) still owes that frame's host half: applying its feedback
+// lists to the cache's LRU and filling its statistics. The cache and the
+// tracer's feedback extractor point at such a tracer, and every entry point
+// that reads or changes their state finishes the frame first.
+int FlushPending(rt_tracer* tracer);
 
 }  // namespace rayt
 

@@ -23,7 +23,8 @@ struct rt_tracer {
   int max_passes = 0;
   int flags = 0;
   rt_buf* out_image = nullptr;
-  rt_buf* codes = nullptr;
+  rt_buf* codes = nullptr;          // the last frame's codes: one of codes_buf
+  rt_buf* codes_buf[2] = {nullptr, nullptr};
   rt_buf* states = nullptr;
   rt_feedback* feedback = nullptr;
   uint64_t stats[5] = {0, 0, 0, 0, 0};
@@ -32,6 +33,19 @@ struct rt_tracer {
   // summed device time between them.
   std::vector<hipEvent_t> ev;
   double trace_ms = 0.0, frame_ms = 0.0;
+  double trace_ms_total = 0.0;  // over every finished frame
+  uint64_t frames_total = 0;
+  // Pipelined frames (fully resident pools): the frame's trace and feedback
+  // walk are queued and the call returns; the host half (the LRU marks of its
+  // hit lists, its statistics) runs when the next frame's trace is queued
+  // behind it, or when anything reads the state (FlushPending). Two code
+  // buffers alternate, so the sort path of a pending frame's feedback still
+  // finds its codes while the next frame writes the other buffer.
+  bool pipelining = true;
+  bool pending = false;
+  int pending_buf = 0;
+  std::chrono::steady_clock::time_point pending_t0;
+  hipEvent_t pev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // trace start / end per code buffer
   hipEvent_t Event(size_t i) {
     while (ev.size() <= i) {
       hipEvent_t e;
@@ -46,11 +60,10 @@ namespace {
 
 const float kBackground[4] = {0.0f, float(216 / 255.), 1.0f, 1.0f};  // gpu-ray-tracer.cpp:190
 
-int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premarked, bool& done) {
+// The transaction of one pass over the extractor's lists (gpu-ray-tracer.cpp:137-186).
+int Transaction(rt_tracer* t, rayt::BlockCacheManager& m, bool& done) {
   rayt::FeedbackExtractor& fx = *t->feedback->fx;
-  int rc = fx.ProcessN(static_cast<int*>(t->codes->ptr), fx.count(), m.nodes_in_block(), true, premarked);
-  if (rc) return rc;
-  rc = m.StartRequestTransaction();
+  int rc = m.StartRequestTransaction();
   if (rc) return rc;
   for (int h : fx.hit_blocks())
     if ((rc = m.MarkBlockAsUsed(h))) return rc;
@@ -72,7 +85,93 @@ int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premar
   return RT_OK;
 }
 
+int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premarked, bool& done) {
+  rayt::FeedbackExtractor& fx = *t->feedback->fx;
+  const int rc = fx.ProcessN(static_cast<int*>(t->codes->ptr), fx.count(), m.nodes_in_block(), true, premarked);
+  return rc ? rc : Transaction(t, m, done);
+}
+
+hipEvent_t PipelineEvent(rt_tracer* t, int buf, int which) {
+  hipEvent_t& e = t->pev[buf][which];
+  if (!e && hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+  return e;
+}
+
+// The host half of a pipelined frame: its feedback lists, one transaction, the
+// end of its session, its statistics and timing.
+int CompletePipelined(rt_tracer* t) {
+  t->pending = false;
+  if (t->cache->pending == t) t->cache->pending = nullptr;
+  t->feedback->pending = nullptr;
+  rayt::BlockCacheManager& m = *t->cache->mgr;
+  int rc = t->feedback->fx->CompleteN();
+  if (rc) return rc;
+  bool done = false;
+  rc = Transaction(t, m, done);
+  if (rc) return rc;
+  t->stats[0] = 1;
+  t->stats[3] = m.SessionFilledCache() ? 1 : 0;
+  t->stats[4] = uint64_t(t->width) * t->height;
+  rc = m.EndRequestSession();
+  if (rc) return rc;
+  float ms = 0.f;
+  RT_HIP_CHECK(hipEventSynchronize(t->pev[t->pending_buf][1]));
+  RT_HIP_CHECK(hipEventElapsedTime(&ms, t->pev[t->pending_buf][0], t->pev[t->pending_buf][1]));
+  t->trace_ms = ms;
+  t->trace_ms_total += ms;
+  t->frames_total += 1;
+  t->frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t->pending_t0).count();
+  return RT_OK;
+}
+
+// A frame over a fully resident pool: the single pass (the fused renderer,
+// marking its codes for the feedback) and the feedback walk are queued; the
+// previous pipelined frame is finished on the host while this trace runs.
+int RenderPipelined(rt_tracer* t, const float vpi[16]) {
+  const auto t0 = std::chrono::steady_clock::now();
+  rayt::BlockCacheManager& m = *t->cache->mgr;
+  rt_cache* c = t->cache;
+  const int buf = t->pending ? 1 - t->pending_buf : t->pending_buf;
+  rt_buf* codes = t->codes_buf[buf];
+  rayt::FeedbackExtractor& fx = *t->feedback->fx;
+  const rayt::MarkTargets targets =
+      fx.Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
+  RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 0), t->ctx->stream));
+  int rc = rayt::RenderPlain(t->ctx, t->out_image, codes, t->width, t->height, rt_cache_channel(c, 0),
+                             rt_cache_far_pointers(c), m.root_node_index(), rt_cache_channel_by_name(c, "normals"),
+                             rt_cache_channel_by_name(c, "colors"), m.nodes_in_block(), vpi, t->lod_voxel_size,
+                             kBackground, t->flags, nullptr, &targets);
+  if (rc) return rc;
+  RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 1), t->ctx->stream));
+  if (t->pending && (rc = CompletePipelined(t))) return rc;
+  for (uint64_t& x : t->stats) x = 0;
+  t->pass_counts.clear();
+  t->pass_faults.clear();
+  t->codes = codes;
+  rc = m.StartRequestSession();
+  if (rc) return rc;
+  rc = fx.EnqueueN(static_cast<int*>(codes->ptr), fx.count(), m.nodes_in_block(), true, true);
+  if (rc) return rc;
+  t->pending = true;
+  t->pending_buf = buf;
+  t->pending_t0 = t0;
+  c->pending = t;
+  t->feedback->pending = t;
+  return RT_OK;
+}
+
 }  // namespace
+
+namespace rayt {
+
+int FlushPending(rt_tracer* t) {
+  if (!t || !t->pending) return RT_OK;
+  RT_TRY_BEGIN
+  return CompletePipelined(t);
+  RT_TRY_END
+}
+
+}  // namespace rayt
 
 extern "C" {
 
@@ -87,7 +186,9 @@ int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
   t->height = height;
   const uint64_t P = uint64_t(width) * height;
   int rc = rt_buf_create(t->ctx, P * 4, &t->out_image);
-  if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes);
+  if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes_buf[0]);
+  if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes_buf[1]);
+  t->codes = t->codes_buf[0];
   if (!rc) rc = rt_buf_create(t->ctx, P * RT_STATE_BYTES_PER_PIXEL, &t->states);
   if (!rc) rc = rt_feedback_create(t->ctx, int(P), &t->feedback);
   if (rc) {
@@ -100,11 +201,17 @@ int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
 
 int rt_tracer_destroy(rt_tracer* t) {
   if (!t) return RT_OK;
+  (void)rayt::FlushPending(t);
+  if (t->cache && t->cache->pending == t) t->cache->pending = nullptr;
   rt_buf_destroy(t->out_image);
-  rt_buf_destroy(t->codes);
+  rt_buf_destroy(t->codes_buf[0]);
+  rt_buf_destroy(t->codes_buf[1]);
   rt_buf_destroy(t->states);
   rt_feedback_destroy(t->feedback);
   for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+  for (auto& pair : t->pev)
+    for (hipEvent_t e : pair)
+      if (e) (void)hipEventDestroy(e);
   rt_ctx* ctx = t->ctx;
   delete t;
   rayt::CtxRelease(ctx);
@@ -136,6 +243,7 @@ int rt_tracer_set_flags(rt_tracer* t, int flags) {
 }
 int rt_tracer_set_seed(rt_tracer* t, uint64_t seed) {
   RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  if (const int rc = rayt::FlushPending(t)) return rc;  // the pending frame draws with the old seed
   t->feedback->fx->set_seed(seed);
   return RT_OK;
 }
@@ -143,6 +251,18 @@ int rt_tracer_set_seed(rt_tracer* t, uint64_t seed) {
 int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
   RT_REQUIRE(t && vpi, RT_ERR_INVALID, "null argument");
   RT_TRY_BEGIN
+  if (t->cache->pending && t->cache->pending != t) {  // another tracer's frame on this cache comes first
+    const int rc = rayt::FlushPending(t->cache->pending);
+    if (rc) return rc;
+  }
+  {
+    rayt::BlockCacheManager& m = *t->cache->mgr;
+    const bool resident = uint32_t(m.cache().loaded_blocks_count()) == m.source()->header().blocks_count;
+    if (resident && t->pipelining) return RenderPipelined(t, vpi);
+    const int rc = rayt::FlushPending(t);
+    if (rc) return rc;
+    t->codes = t->codes_buf[t->pending_buf];
+  }
   const auto t0 = std::chrono::steady_clock::now();
   rayt::BlockCacheManager& m = *t->cache->mgr;
   rt_cache* c = t->cache;
@@ -204,13 +324,32 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
     RT_HIP_CHECK(hipEventElapsedTime(&ms, t->ev[2 * i], t->ev[2 * i + 1]));
     t->trace_ms += ms;
   }
+  t->trace_ms_total += t->trace_ms;
+  t->frames_total += 1;
   t->frame_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return RT_OK;
   RT_TRY_END
 }
 
+int rt_tracer_set_pipelining(rt_tracer* t, int enabled) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  if (!enabled)
+    if (const int rc = rayt::FlushPending(t)) return rc;
+  t->pipelining = enabled != 0;
+  return RT_OK;
+}
+
+int rt_tracer_trace_totals(rt_tracer* t, double* trace_ms, uint64_t* frames) {
+  RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  if (const int rc = rayt::FlushPending(t)) return rc;
+  if (trace_ms) *trace_ms = t->trace_ms_total;
+  if (frames) *frames = t->frames_total;
+  return RT_OK;
+}
+
 int rt_tracer_timings(const rt_tracer* t, double out_ms[2]) {
   RT_REQUIRE(t && out_ms, RT_ERR_INVALID, "null argument");
+  if (const int rc = rayt::FlushPending(const_cast<rt_tracer*>(t))) return rc;
   out_ms[0] = t->trace_ms;
   out_ms[1] = t->frame_ms;
   return RT_OK;
@@ -222,12 +361,14 @@ rt_feedback* rt_tracer_feedback(rt_tracer* t) { return t ? t->feedback : nullptr
 
 int rt_tracer_frame_stats(const rt_tracer* t, uint64_t out[5]) {
   RT_REQUIRE(t && out, RT_ERR_INVALID, "null argument");
+  if (const int rc = rayt::FlushPending(const_cast<rt_tracer*>(t))) return rc;
   memcpy(out, t->stats, sizeof(t->stats));
   return RT_OK;
 }
 
 int rt_tracer_pass_faults(const rt_tracer* t, int* counts, int counts_cap, int* faults, int faults_cap) {
   RT_REQUIRE(t, RT_ERR_INVALID, "null tracer");
+  if (const int rc = rayt::FlushPending(const_cast<rt_tracer*>(t))) return rc;
   for (int i = 0; i < counts_cap && i < int(t->pass_counts.size()); ++i) counts[i] = t->pass_counts[i];
   for (int i = 0; i < faults_cap && i < int(t->pass_faults.size()); ++i) faults[i] = t->pass_faults[i];
   return int(t->pass_counts.size());

@@ -71,3 +71,60 @@ def test_frame_resident_single_pass(gpu_ctx):
     """Whole tree resident: one pass (fused kernel), no faults, LRU touched by hits."""
     _run(gpu_ctx, depth=8, nib=8192, boxes=0, pool_frac=1.0, W=320, H=240, flags=api.L.RT_FLAG_SHADOWS,
          max_passes=0, lod=0.0, poses=POSES[:3])
+
+
+def _resident_run(gpu_ctx, tree, pipelined, W, H, poses, query_at=None):
+    info = tree.info()
+    m = api.GPUOctreeCacheManager(info.blocks_count, tree, gpu_ctx)
+    m.InitialFillCache()
+    tr = api.GPURayTracer(m, W, H)
+    tr.set_pipelining(pipelined)
+    tr.set_frame_time_limit(1e9)
+    tr.set_lod_voxel_size(0.0)
+    tr.set_flags(api.L.RT_FLAG_SHADOWS)
+    tr.set_seed(5)
+    mid = None
+    for k, (yaw, pitch, pos) in enumerate(poses):
+        tr.RenderFrame(camera_vpi(yaw, pitch, pos, aspect=W / H))
+        if k == query_at:  # through the cache and the extractor: a pending frame is finished first
+            mid = (m.lru_order(), tr.feedback().hit_blocks(), m.counters())
+    fb = tr.feedback()
+    return dict(mid=mid, lru=m.lru_order(), stats=tr.frame_stats(), faults=tr.pass_faults(), image=tr.out_image(),
+                codes=tr.codes(), frames=tr.trace_totals()[1], hits=fb.hit_blocks(), dups=fb.duplicate_hit_blocks(),
+                seed_state=fb.stats())
+
+
+def test_pipelined_resident_frames_equal_unpipelined(gpu_ctx):
+    """Frames queued back to back (the host half of each finished behind the next trace) leave exactly the
+    state of frames rendered to the end one at a time: LRU order, lists, statistics, image and codes."""
+    tree, _ = terrain_tree(8, 4096, 0, 8)
+    W, H = 320, 240
+    poses = POSES + POSES[:3]
+    a = _resident_run(gpu_ctx, tree, True, W, H, poses, query_at=3)
+    b = _resident_run(gpu_ctx, tree, False, W, H, poses, query_at=3)
+    assert a["frames"] == b["frames"] == len(poses)
+    for key in ("lru", "stats", "faults", "hits", "dups", "seed_state"):
+        assert a[key] == b[key], key
+    assert a["mid"][0] == b["mid"][0] and a["mid"][1] == b["mid"][1] and a["mid"][2] == b["mid"][2]
+    np.testing.assert_array_equal(a["image"], b["image"])
+    np.testing.assert_array_equal(a["codes"], b["codes"])
+    assert len(a["hits"]) > 0
+
+
+def test_two_tracers_on_one_cache_take_turns(gpu_ctx):
+    """A second tracer's frame on the same cache finishes the first one's pending frame before its own."""
+    tree, _ = terrain_tree(8, 4096, 0, 8)
+    info = tree.info()
+    orders = []
+    for pipelined in (True, False):
+        m = api.GPUOctreeCacheManager(info.blocks_count, tree, gpu_ctx)
+        m.InitialFillCache()
+        t1, t2 = api.GPURayTracer(m, 200, 150), api.GPURayTracer(m, 120, 90)
+        for tr in (t1, t2):
+            tr.set_pipelining(pipelined)
+            tr.set_lod_voxel_size(0.0)
+            tr.set_seed(3)
+        for k, (yaw, pitch, pos) in enumerate(POSES):
+            (t1 if k % 2 == 0 else t2).RenderFrame(camera_vpi(yaw, pitch, pos))
+        orders.append((m.lru_order(), t1.frame_stats(), t2.frame_stats()))
+    assert orders[0] == orders[1]
