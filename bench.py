@@ -528,7 +528,7 @@ def main():
     ap.add_argument("--obj-depth", type=int, default=11)
     ap.add_argument("--obj-grid", type=int, default=256, help="heightfield mesh of 2 * grid^2 triangles")
     ap.add_argument("--dump", default="", metavar="DIR", help="write the measured frame and its feedback codes as PNG")
-    ap.add_argument("--plain-shape", default="4,10", metavar="WAVES,LEVELS|sweep",
+    ap.add_argument("--plain-shape", default="1,10", metavar="WAVES,LEVELS|sweep",
                     help="plain kernel launch shape (the one the committed HBM-traffic profile measured), or "
                          "'sweep' to time 1/4 waves x 10/12/16 LDS levels and keep the fastest")
     ap.add_argument("--batch-shape", default="", metavar="TILE,LEVELS,WGS",
