@@ -49,13 +49,36 @@ rt_ctx* CtxRetain(rt_ctx* ctx);
 void CtxRelease(rt_ctx* ctx);
 
 struct MarkTargets;
+
+// Tile order of a plain launch, for a caller that renders the same view size
+// again and again (the tracer). Each wave stores its duration in cycles in
+// `cost` (waves of tile t at t * waves_per_tile); after the launch a
+// one-workgroup kernel writes into `order` the next launch's tiles, the
+// longest (from this launch's costs) first (OrderTiles), so that the frame
+// does not end on a long tile started last. order_valid = false: this launch
+// takes the tiles in raster order. The output does not depend on the order.
+constexpr int kMaxScheduledTiles = 64000;  // larger views keep the raster order
+struct TileSchedule {
+  uint32_t* order = nullptr;  // one entry per tile
+  uint32_t* cost = nullptr;   // one entry per wave
+  bool order_valid = false;
+};
+
 // rt_render_plain, and with `mark` also the marking of its block-level codes
 // into a feedback extractor's bitmaps (FeedbackExtractor::Targets), so that
-// the extractor's Process can skip its own pass over the codes.
+// the extractor's Process can skip its own pass over the codes; with `sched`
+// the tiles are scheduled as TileSchedule says.
 int RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, int height, const rt_buf* node_links,
                 const rt_buf* far_pointers, int root_node_index, const rt_buf* normals, const rt_buf* colors,
                 int nodes_in_block, const float vpi[16], float lod_voxel_size, const float background[4], int flags,
-                uint64_t* stats, const MarkTargets* mark);
+                uint64_t* stats, const MarkTargets* mark, const TileSchedule* sched = nullptr);
+
+// Writes sched.order for the next launch from the costs a RenderPlain with
+// `sched` just stored (queued on the context stream after it).
+int OrderTiles(rt_ctx* ctx, const TileSchedule& sched, int width, int height);
+
+// Tiles of one plain launch at the context's launch shape, and the workgroup size in waves.
+void PlainTiles(const rt_ctx* ctx, int width, int height, uint32_t* tiles, int* waves_per_tile);
 
 // A tracer whose last frame was pipelined (rt_tracer_render_frame on a fully
 // resident pool) still owes that frame's host half: applying its feedback

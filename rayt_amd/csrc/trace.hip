@@ -13,6 +13,7 @@
 #include <cstring>
 #include <vector>
 
+#include "block_scan.h"
 #include "mark_codes.h"
 #include "rt_internal.h"
 #include "svo_device.h"
@@ -161,17 +162,21 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
 // its LDS stack at once and more waves fit a CU.
 // mark.fault_bits != nullptr: each wave also marks its (block-level) codes
 // into the feedback extractor's bitmaps.
+// order / cost (TileSchedule): the tile of workgroup i is order[i] (raster
+// order without one), and each wave stores its duration in cycles at
+// cost[tile * WAVES + wave].
 template <int WAVES, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
 __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
     uint32_t* __restrict__ out, int* __restrict__ codes, int W, int H, PoolView pool, uint32_t root,
     const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors, int nodes_in_block, Mat16 m,
     float lod, Vec4 bg, int flags, Lights lt, int lds_base, unsigned long long* __restrict__ stats,
-    rayt::MarkTargets mark) {
+    rayt::MarkTargets mark, const uint32_t* __restrict__ order, uint32_t* __restrict__ cost) {
   constexpr int BLOCK = 64 * WAVES;
   constexpr int TS = WAVES == 4 ? 16 : 8;
   __shared__ uint2 lds[LDS_LEVELS * BLOCK];
+  const uint64_t t_start = cost ? clock64() : 0;
   const uint32_t tiles_x = (W + TS - 1) / TS;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
   const int x = int((tile % tiles_x) * TS + (wave & 1u) * 8u + (lane & 7u));
   const int y = int((tile / tiles_x) * TS + (wave >> 1) * 8u + (lane >> 3));
@@ -194,6 +199,99 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
   }
   if (mark.fault_bits) rayt::MarkWaveCodes(code, inside, mark);
   if (COUNT) FlushCounters(cnt, stats);
+  if (cost && lane == 0) cost[tile * WAVES + wave] = uint32_t(min(clock64() - t_start, uint64_t(0xffffffffu)));
+}
+
+// The next launch's tile order from this launch's wave costs: the tiles in
+// four classes by their longest wave (within a factor 2, 4, 8 of the longest
+// tile's, then the rest; compared on a log scale of 1/8 octave), longest
+// class first, raster order within a class. One workgroup of 16 waves; wave
+// w owns a contiguous run of tiles, reads their costs once (sixteen loads in
+// flight per lane, unconditional: an index past the run reads its last tile)
+// into one log-cost byte per tile in LDS, and places the tiles by ballots.
+constexpr int kOrderWaves = 16;
+constexpr int kOrderMaxTiles = rayt::kMaxScheduledTiles;  // LDS holds one byte per tile
+
+__device__ __forceinline__ uint32_t LogCost(uint32_t c) {  // 8 x log2(c), from the exponent and 3 mantissa bits
+  if (c < 16u) return c >> 1;
+  const uint32_t e = 31u - __clz(c);
+  return e * 8u + ((c >> (e - 3u)) & 7u);
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * kOrderWaves) void TileOrderKernel(const uint32_t* __restrict__ cost, int tiles,
+                                                                    uint32_t* __restrict__ order) {
+  constexpr int kLoads = 16;
+  __shared__ uint8_t lc[kOrderMaxTiles];
+  __shared__ uint32_t counts[kOrderWaves][4];
+  __shared__ uint32_t top;
+  const int w = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+  if (threadIdx.x == 0) top = 0;
+  __syncthreads();
+  const int chunk = (((tiles + kOrderWaves - 1) / kOrderWaves) + 63) & ~63;
+  const int b = min(tiles, w * chunk), e = min(tiles, b + chunk);
+  uint32_t mx = 0;
+  for (int t0 = b; t0 < e; t0 += 64 * kLoads) {
+    uint32_t c[kLoads];
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+      const int t = min(t0 + 64 * k + lane, e - 1);
+      c[k] = cost[t * WAVES];
+#pragma unroll
+      for (int v = 1; v < WAVES; ++v) c[k] = max(c[k], cost[t * WAVES + v]);
+    }
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+      const int t = t0 + 64 * k + lane;
+      if (t < e) {
+        const uint32_t l = LogCost(c[k]);
+        lc[t] = uint8_t(l);
+        mx = max(mx, l);
+      }
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) mx = max(mx, uint32_t(__shfl_xor(int(mx), o, 64)));
+  if (lane == 0) atomicMax(&top, mx);
+  __syncthreads();
+  const uint32_t T = top;
+  auto cls = [&](int t) {
+    const uint32_t d = T - uint32_t(lc[t]);
+    return d < 8u ? 0 : d < 16u ? 1 : d < 24u ? 2 : 3;
+  };
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  for (int t0 = b; t0 < e; t0 += 64) {
+    const int t = t0 + lane;
+    const int c = t < e ? cls(t) : 4;
+    n0 += __popcll(__ballot(c == 0));
+    n1 += __popcll(__ballot(c == 1));
+    n2 += __popcll(__ballot(c == 2));
+    n3 += __popcll(__ballot(c == 3));
+  }
+  if (lane == 0) counts[w][0] = n0, counts[w][1] = n1, counts[w][2] = n2, counts[w][3] = n3;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // class-major, then wave (= tile) order: starting positions
+    uint32_t run = 0;
+    for (int k = 0; k < 4; ++k)
+      for (int v = 0; v < kOrderWaves; ++v) {
+        const uint32_t m = counts[v][k];
+        counts[v][k] = run;
+        run += m;
+      }
+  }
+  __syncthreads();
+  uint32_t base[4] = {counts[w][0], counts[w][1], counts[w][2], counts[w][3]};
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  for (int t0 = b; t0 < e; t0 += 64) {
+    const int t = t0 + lane;
+    const int c = t < e ? cls(t) : 4;
+    const uint64_t m0 = __ballot(c == 0), m1 = __ballot(c == 1), m2 = __ballot(c == 2), m3 = __ballot(c == 3);
+    const uint64_t mine = c == 0 ? m0 : c == 1 ? m1 : c == 2 ? m2 : m3;
+    if (t < e) order[base[c] + uint32_t(__popcll(mine & below))] = uint32_t(t);
+    base[0] += __popcll(m0);
+    base[1] += __popcll(m1);
+    base[2] += __popcll(m2);
+    base[3] += __popcll(m3);
+  }
 }
 
 // ----------------------------------------------------- RaytraceKernel (depth)
@@ -353,16 +451,17 @@ template <int WV, int L, bool C, bool SP>
 void LaunchPlainL(dim3 g, hipStream_t s, uint32_t* out, int* codes, int W, int H, const PoolView& pool,
                   uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib, const Mat16& m, float lod,
                   const Vec4& bg, int flags, const Lights& lt, int base, unsigned long long* st,
-                  const rayt::MarkTargets& mk) {
+                  const rayt::MarkTargets& mk, const uint32_t* order, uint32_t* cost) {
   if constexpr (!C) {
     if (lod == 0.0f) {
       RenderPlainKernel<WV, L, C, SP, false><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib,
-                                                                         m, lod, bg, flags, lt, base, st, mk);
+                                                                         m, lod, bg, flags, lt, base, st, mk, order,
+                                                                         cost);
       return;
     }
   }
   RenderPlainKernel<WV, L, C, SP, true><<<g, dim3(64 * WV), 0, s>>>(out, codes, W, H, pool, root, nrm, col, nib, m,
-                                                                    lod, bg, flags, lt, base, st, mk);
+                                                                    lod, bg, flags, lt, base, st, mk, order, cost);
 }
 
 // spill = false selects the window-only stack (the tree fits the LDS window).
@@ -370,17 +469,18 @@ template <bool C>
 bool DispatchPlain(int waves, int levels, bool spill, hipStream_t s, uint32_t* out, int* codes, int W, int H,
                    const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col, int nib,
                    const Mat16& m, float lod, const Vec4& bg, int flags, const Lights& lt, int base,
-                   unsigned long long* st, const rayt::MarkTargets& mk) {
+                   unsigned long long* st, const rayt::MarkTargets& mk, const uint32_t* order = nullptr,
+                   uint32_t* cost = nullptr) {
   const int ts = waves == 4 ? 16 : 8;
   const dim3 g(uint32_t((W + ts - 1) / ts) * uint32_t((H + ts - 1) / ts));
 #define RT_CASE(WV, L)                                                                                      \
   if (waves == WV && levels == L) {                                                                         \
     if (spill || L >= kStackLevels)                                                                         \
       LaunchPlainL<WV, L, C, true>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod, bg, flags, lt, \
-                                   base, st, mk);                                                           \
+                                   base, st, mk, order, cost);                                              \
     else                                                                                                    \
       LaunchPlainL<WV, L, C, L >= kStackLevels>(g, s, out, codes, W, H, pool, root, nrm, col, nib, m, lod,  \
-                                                bg, flags, lt, base, st, mk);                               \
+                                                bg, flags, lt, base, st, mk, order, cost);                  \
     return true;                                                                                            \
   }
   RT_CASE(1, 8) RT_CASE(1, 10) RT_CASE(1, 12) RT_CASE(1, 14) RT_CASE(1, 16) RT_CASE(1, 22)
@@ -535,7 +635,7 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
                       const rt_buf* node_links, const rt_buf* far_pointers, int root_node_index,
                       const rt_buf* normals, const rt_buf* colors, int nodes_in_block, const float vpi[16],
                       float lod_voxel_size, const float background[4], int flags, uint64_t* stats,
-                      const MarkTargets* mark) {
+                      const MarkTargets* mark, const TileSchedule* sched) {
   int rc = CheckTraceArgs(ctx, width, height);
   if (rc) return rc;
   RT_REQUIRE(out_image && node_links && far_pointers && vpi && background, RT_ERR_INVALID, "null argument");
@@ -546,6 +646,8 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
   RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
   RT_REQUIRE(!mark || (codes && nodes_in_block > 0 && !(flags & RT_FLAG_COUNT)), RT_ERR_INVALID,
              "marking needs block-level codes and the timing variant");
+  RT_REQUIRE(!sched || (sched->order && sched->cost && !(flags & RT_FLAG_COUNT)), RT_ERR_INVALID,
+             "a tile schedule needs its buffers and the timing variant");
   const MarkTargets no_marks;
   const MarkTargets& mk = mark ? *mark : no_marks;
   Mat16 m;
@@ -580,7 +682,28 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
                        static_cast<uint32_t*>(out_image->ptr),
                        codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
                        uint32_t(root_node_index), U32(normals), U32(colors), nodes_in_block, m,
-                       lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr, mk);
+                       lod_voxel_size, bg, flags, lt, ctx->plain_lds_base, nullptr, mk,
+                       sched && sched->order_valid ? sched->order : nullptr, sched ? sched->cost : nullptr);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
+}
+
+int rayt::OrderTiles(rt_ctx* ctx, const TileSchedule& sched, int width, int height) {
+  uint32_t tiles;
+  int waves;
+  PlainTiles(ctx, width, height, &tiles, &waves);
+  RT_REQUIRE(tiles <= uint32_t(kOrderMaxTiles), RT_ERR_RANGE, "too many tiles to schedule");
+  if (waves == 4)
+    TileOrderKernel<4><<<dim3(1), dim3(64 * kOrderWaves), 0, ctx->stream>>>(sched.cost, int(tiles), sched.order);
+  else
+    TileOrderKernel<1><<<dim3(1), dim3(64 * kOrderWaves), 0, ctx->stream>>>(sched.cost, int(tiles), sched.order);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+void rayt::PlainTiles(const rt_ctx* ctx, int width, int height, uint32_t* tiles, int* waves_per_tile) {
+  const int waves = ctx->plain_waves == 4 ? 4 : 1;
+  const int ts = waves == 4 ? 16 : 8;
+  *tiles = uint32_t((width + ts - 1) / ts) * uint32_t((height + ts - 1) / ts);
+  *waves_per_tile = waves;
 }

@@ -6,6 +6,7 @@
 // cycled the whole cache, or the pass budget is spent; FinishTracingFrame
 // then composes the image. When the cache holds every block of the tree no
 // ray can fault, and the single pass plus finish run as one fused kernel.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <limits>
@@ -25,6 +26,13 @@ struct rt_tracer {
   rt_buf* out_image = nullptr;
   rt_buf* codes = nullptr;          // the last frame's codes: one of codes_buf
   rt_buf* codes_buf[2] = {nullptr, nullptr};
+  // Resident frames schedule their tiles longest first from the previous
+  // frame's wave costs (TileSchedule); sched_waves is the launch shape the
+  // order was made for.
+  rt_buf* tile_cost = nullptr;
+  rt_buf* tile_order = nullptr;
+  int sched_waves = 0;
+  uint64_t sched_frames = 0;  // resident frames since the order was last rebuilt
   rt_buf* states = nullptr;
   rt_feedback* feedback = nullptr;
   uint64_t stats[5] = {0, 0, 0, 0, 0};
@@ -91,6 +99,39 @@ int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premar
   return rc ? rc : Transaction(t, m, done);
 }
 
+// Resident frames between rebuilds of the tile order: the rebuild (one
+// workgroup, about 20 us) follows the first frame of a launch shape and then
+// every kReorderFrames-th frame; the frames between keep the order.
+constexpr uint64_t kReorderFrames = 8;
+
+// The tile schedule of the next resident frame (its order is valid when the
+// previous resident frame had the same launch shape).
+rayt::TileSchedule Schedule(rt_tracer* t) {
+  rayt::TileSchedule s;
+  uint32_t tiles;
+  int waves;
+  rayt::PlainTiles(t->ctx, t->width, t->height, &tiles, &waves);
+  s.order = static_cast<uint32_t*>(t->tile_order->ptr);
+  s.cost = static_cast<uint32_t*>(t->tile_cost->ptr);
+  s.order_valid = t->sched_waves == waves;
+  if (!s.order_valid) t->sched_frames = 0;
+  t->sched_waves = waves;
+  if (tiles > uint32_t(rayt::kMaxScheduledTiles)) {  // raster order, no costs: nothing to order
+    s.order_valid = false;
+    s.order = nullptr;
+    s.cost = nullptr;
+  }
+  return s;
+}
+
+// Queues the rebuild of the tile order after a resident frame when it is due.
+int MaybeReorder(rt_tracer* t, const rayt::TileSchedule& sched) {
+  if (!sched.cost) return RT_OK;
+  const bool due = t->sched_frames % kReorderFrames == 0;
+  ++t->sched_frames;
+  return due ? rayt::OrderTiles(t->ctx, sched, t->width, t->height) : RT_OK;
+}
+
 hipEvent_t PipelineEvent(rt_tracer* t, int buf, int which) {
   hipEvent_t& e = t->pev[buf][which];
   if (!e && hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
@@ -136,13 +177,15 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
   rayt::FeedbackExtractor& fx = *t->feedback->fx;
   const rayt::MarkTargets targets =
       fx.Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
+  const rayt::TileSchedule sched = Schedule(t);
   RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 0), t->ctx->stream));
   int rc = rayt::RenderPlain(t->ctx, t->out_image, codes, t->width, t->height, rt_cache_channel(c, 0),
                              rt_cache_far_pointers(c), m.root_node_index(), rt_cache_channel_by_name(c, "normals"),
                              rt_cache_channel_by_name(c, "colors"), m.nodes_in_block(), vpi, t->lod_voxel_size,
-                             kBackground, t->flags, nullptr, &targets);
+                             kBackground, t->flags, nullptr, &targets, sched.cost ? &sched : nullptr);
   if (rc) return rc;
   RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 1), t->ctx->stream));
+  if ((rc = MaybeReorder(t, sched))) return rc;
   if (t->pending && (rc = CompletePipelined(t))) return rc;
   for (uint64_t& x : t->stats) x = 0;
   t->pass_counts.clear();
@@ -188,6 +231,13 @@ int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
   int rc = rt_buf_create(t->ctx, P * 4, &t->out_image);
   if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes_buf[0]);
   if (!rc) rc = rt_buf_create(t->ctx, P * 4, &t->codes_buf[1]);
+  {
+    // entries for either launch shape: 8x8 tiles of one wave, 16x16 tiles of four
+    const uint64_t t8 = uint64_t((width + 7) / 8) * uint64_t((height + 7) / 8);
+    const uint64_t t16 = uint64_t((width + 15) / 16) * uint64_t((height + 15) / 16);
+    if (!rc) rc = rt_buf_create(t->ctx, std::max(t8, 4 * t16) * 4, &t->tile_cost);
+    if (!rc) rc = rt_buf_create(t->ctx, std::max(t8, t16) * 4, &t->tile_order);
+  }
   t->codes = t->codes_buf[0];
   if (!rc) rc = rt_buf_create(t->ctx, P * RT_STATE_BYTES_PER_PIXEL, &t->states);
   if (!rc) rc = rt_feedback_create(t->ctx, int(P), &t->feedback);
@@ -206,6 +256,8 @@ int rt_tracer_destroy(rt_tracer* t) {
   rt_buf_destroy(t->out_image);
   rt_buf_destroy(t->codes_buf[0]);
   rt_buf_destroy(t->codes_buf[1]);
+  rt_buf_destroy(t->tile_cost);
+  rt_buf_destroy(t->tile_order);
   rt_buf_destroy(t->states);
   rt_feedback_destroy(t->feedback);
   for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
@@ -290,15 +342,19 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
     const bool premarked = resident && iterations == 0;
     const rayt::MarkTargets targets =
         t->feedback->fx->Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
-    if (resident)
+    rayt::TileSchedule sched;
+    if (resident) {
+      sched = Schedule(t);
       rc = rayt::RenderPlain(t->ctx, t->out_image, t->codes, t->width, t->height, links, far_ptrs,
                              m.root_node_index(), normals, colors, nib, vpi, t->lod_voxel_size, kBackground,
-                             t->flags, nullptr, premarked ? &targets : nullptr);
+                             t->flags, nullptr, premarked ? &targets : nullptr, sched.cost ? &sched : nullptr);
+    }
     else
       rc = rt_raytracing_pass(t->ctx, t->codes, t->states, t->width, t->height, links, far_ptrs, m.root_node_index(),
                               normals, colors, nib, t->flags);
     if (rc) return rc;
     RT_HIP_CHECK(hipEventRecord(t->Event(2 * size_t(iterations) + 1), t->ctx->stream));
+    if (resident && (rc = MaybeReorder(t, sched))) return rc;
     if (iterations > 0) {
       const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if ((t->max_passes > 0 && iterations + 1 >= t->max_passes) || el > t->frame_time_limit) break;

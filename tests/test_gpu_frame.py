@@ -128,3 +128,28 @@ def test_two_tracers_on_one_cache_take_turns(gpu_ctx):
             (t1 if k % 2 == 0 else t2).RenderFrame(camera_vpi(yaw, pitch, pos))
         orders.append((m.lru_order(), t1.frame_stats(), t2.frame_stats()))
     assert orders[0] == orders[1]
+
+
+def test_scheduled_tiles_render_what_raster_order_renders(gpu_ctx):
+    """Resident frames take their tiles longest first from the previous frame's wave costs; every frame,
+    across launch-shape changes (which restart from raster order), equals the stateless plain render."""
+    tree, _ = terrain_tree(8, 4096, 0, 8)
+    info = tree.info()
+    m = api.GPUOctreeCacheManager(info.blocks_count, tree, gpu_ctx)
+    m.InitialFillCache()
+    W, H = 328, 200  # partial tiles at the right and bottom edges
+    tr = api.GPURayTracer(m, W, H)
+    tr.set_lod_voxel_size(0.0)
+    tr.set_flags(api.L.RT_FLAG_SHADOWS)
+    try:
+        for k, (waves, levels) in enumerate([(1, 10), (1, 10), (1, 10), (4, 10), (4, 10), (1, 12), (1, 12)]):
+            api.plain_config(gpu_ctx, levels, waves, 0)
+            yaw, pitch, pos = POSES[k % len(POSES)]
+            vpi = camera_vpi(yaw, pitch, pos, aspect=W / H)
+            tr.RenderFrame(vpi)
+            img, codes = tr.out_image(), tr.codes()
+            out, cod, _ = api.render_plain(gpu_ctx, m, W, H, vpi, lod=0.0, flags=api.L.RT_FLAG_SHADOWS)
+            np.testing.assert_array_equal(img, out.read_all(np.uint32).reshape(H, W), err_msg=f"frame {k}")
+            np.testing.assert_array_equal(codes, cod.read_all(np.int32).reshape(H, W), err_msg=f"codes, frame {k}")
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
