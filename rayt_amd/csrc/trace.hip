@@ -203,14 +203,15 @@ __global__ __launch_bounds__(64 * WAVES) void RenderPlainKernel(
 }
 
 // The next launch's tile order from this launch's wave costs: the tiles in
-// four classes by their longest wave (within a factor 2, 4, 8 of the longest
-// tile's, then the rest; compared on a log scale of 1/8 octave), longest
-// class first, raster order within a class. One workgroup of 16 waves; wave
+// eight classes by their longest wave (half an octave each below the longest
+// tile's, the last class open-ended; costs on a log scale of 1/8 octave),
+// longest class first, raster order within a class. One workgroup of 16 waves; wave
 // w owns a contiguous run of tiles, reads their costs once (sixteen loads in
 // flight per lane, unconditional: an index past the run reads its last tile)
 // into one log-cost byte per tile in LDS, and places the tiles by ballots.
 constexpr int kOrderWaves = 16;
 constexpr int kOrderMaxTiles = rayt::kMaxScheduledTiles;  // LDS holds one byte per tile
+constexpr int kOrderClasses = 8;  // half-octave cost classes, the last one open-ended
 
 __device__ __forceinline__ uint32_t LogCost(uint32_t c) {  // 8 x log2(c), from the exponent and 3 mantissa bits
   if (c < 16u) return c >> 1;
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(64 * kOrderWaves) void TileOrderKernel(const uint32
                                                                     uint32_t* __restrict__ order) {
   constexpr int kLoads = 16;
   __shared__ uint8_t lc[kOrderMaxTiles];
-  __shared__ uint32_t counts[kOrderWaves][4];
+  __shared__ uint32_t counts[kOrderWaves][kOrderClasses];
   __shared__ uint32_t top;
   const int w = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
   if (threadIdx.x == 0) top = 0;
@@ -254,24 +255,24 @@ __global__ __launch_bounds__(64 * kOrderWaves) void TileOrderKernel(const uint32
   if (lane == 0) atomicMax(&top, mx);
   __syncthreads();
   const uint32_t T = top;
-  auto cls = [&](int t) {
-    const uint32_t d = T - uint32_t(lc[t]);
-    return d < 8u ? 0 : d < 16u ? 1 : d < 24u ? 2 : 3;
+  auto cls = [&](int t) {  // half an octave per class below the longest tile
+    const uint32_t d = (T - uint32_t(lc[t])) >> 2;
+    return int(min(d, uint32_t(kOrderClasses - 1)));
   };
-  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  uint32_t n[kOrderClasses] = {};
   for (int t0 = b; t0 < e; t0 += 64) {
     const int t = t0 + lane;
-    const int c = t < e ? cls(t) : 4;
-    n0 += __popcll(__ballot(c == 0));
-    n1 += __popcll(__ballot(c == 1));
-    n2 += __popcll(__ballot(c == 2));
-    n3 += __popcll(__ballot(c == 3));
+    const int c = t < e ? cls(t) : kOrderClasses;
+#pragma unroll
+    for (int k = 0; k < kOrderClasses; ++k) n[k] += __popcll(__ballot(c == k));
   }
-  if (lane == 0) counts[w][0] = n0, counts[w][1] = n1, counts[w][2] = n2, counts[w][3] = n3;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < kOrderClasses; ++k) counts[w][k] = n[k];
   __syncthreads();
   if (threadIdx.x == 0) {  // class-major, then wave (= tile) order: starting positions
     uint32_t run = 0;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kOrderClasses; ++k)
       for (int v = 0; v < kOrderWaves; ++v) {
         const uint32_t m = counts[v][k];
         counts[v][k] = run;
@@ -279,18 +280,21 @@ __global__ __launch_bounds__(64 * kOrderWaves) void TileOrderKernel(const uint32
       }
   }
   __syncthreads();
-  uint32_t base[4] = {counts[w][0], counts[w][1], counts[w][2], counts[w][3]};
+  uint32_t base[kOrderClasses];
+#pragma unroll
+  for (int k = 0; k < kOrderClasses; ++k) base[k] = counts[w][k];
   const uint64_t below = (uint64_t(1) << lane) - 1;
   for (int t0 = b; t0 < e; t0 += 64) {
     const int t = t0 + lane;
-    const int c = t < e ? cls(t) : 4;
-    const uint64_t m0 = __ballot(c == 0), m1 = __ballot(c == 1), m2 = __ballot(c == 2), m3 = __ballot(c == 3);
-    const uint64_t mine = c == 0 ? m0 : c == 1 ? m1 : c == 2 ? m2 : m3;
-    if (t < e) order[base[c] + uint32_t(__popcll(mine & below))] = uint32_t(t);
-    base[0] += __popcll(m0);
-    base[1] += __popcll(m1);
-    base[2] += __popcll(m2);
-    base[3] += __popcll(m3);
+    const int c = t < e ? cls(t) : kOrderClasses;
+    uint32_t pos = 0;
+#pragma unroll
+    for (int k = 0; k < kOrderClasses; ++k) {
+      const uint64_t m = __ballot(c == k);
+      if (c == k) pos = base[k] + uint32_t(__popcll(m & below));
+      base[k] += __popcll(m);
+    }
+    if (t < e) order[pos] = uint32_t(t);
   }
 }
 
