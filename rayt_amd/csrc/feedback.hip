@@ -299,13 +299,14 @@ __global__ __launch_bounds__(kMarkBlock) void MarkCodesKernel(const int* __restr
 // The state and the first prefix_len distinct codes also go straight to
 // pinned host memory (host_state, host_prefix): the host reads them after the
 // kernel, with no copy engine between this kernel and the next one.
-__global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void WalkBitmapsKernel(uint32_t* __restrict__ fault_bits,
                                                           uint32_t* __restrict__ hit_bits, MarkState* __restrict__ st,
                                                           int* __restrict__ unique_out,
                                                           const int4* __restrict__ partials, int nparts,
                                                           int2 premark_words, MarkState* __restrict__ host_state,
                                                           int* __restrict__ host_prefix, int prefix_len) {
-  __shared__ uint32_t waves[1024 / 64 + 1];
+  __shared__ uint32_t waves[BLOCK / 64 + 1];
   __shared__ int red[4];
   const bool premarked = premark_words.y > 0;
   if (threadIdx.x < 4) red[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
     }
   } else {
     int4 acc = make_int4(-1, -1, 0, 0);
-    for (int j = threadIdx.x; j < nparts; j += 1024) {
+    for (int j = threadIdx.x; j < nparts; j += BLOCK) {
       const int4 p = partials[j];
       acc.x = max(acc.x, p.x);
       acc.y = max(acc.y, p.y);
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
   const int wh = premarked ? premark_words.y : max_hit >= 0 ? (max_hit >> 5) + 1 : 0;
   const uint32_t flag_bits = premarked ? (1u << kMarkNoHitKey) | (1u << kMarkOverflowKey) : 0u;
   const int L = wf + wh;
-  const int per = (L + 1023) / 1024;
+  const int per = (L + BLOCK - 1) / BLOCK;
   const int b = threadIdx.x * per, e = min(L, b + per);
   auto word = [&](int j) -> uint32_t {
     return j < wf ? fault_bits[wf - 1 - j] : j == wf ? hit_bits[0] & ~flag_bits : hit_bits[j - wf];
@@ -349,12 +350,12 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
   uint32_t cnt = 0;
   for (int j = b; j < e; ++j) cnt += __popc(word(j));
   uint32_t total;
-  uint32_t pos = BlockExclusiveScan<1024>(cnt, waves, total);
+  uint32_t pos = BlockExclusiveScan<BLOCK>(cnt, waves, total);
   uint32_t faults_total;
   {
     uint32_t fc = 0;
     for (int j = b; j < min(e, wf); ++j) fc += __popc(word(j));
-    BlockExclusiveScan<1024>(fc, waves, faults_total);
+    BlockExclusiveScan<BLOCK>(fc, waves, faults_total);
   }
   for (int j = b; j < e; ++j) {
     uint32_t w = word(j);
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(1024) void WalkBitmapsKernel(uint32_t* __restrict__
   if (host_state) {
     __syncthreads();  // the NO_HIT entry is in place
     const int m = min(u, prefix_len);
-    for (int i = threadIdx.x; i < m; i += 1024) host_prefix[i] = unique_out[i];
+    for (int i = threadIdx.x; i < m; i += BLOCK) host_prefix[i] = unique_out[i];
     if (threadIdx.x == 0) {
       host_state->unique_count = fin.unique_count;
       host_state->fault_count = fin.fault_count;
@@ -563,10 +564,18 @@ int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool bloc
   // The walk writes the state and, speculatively, the head of the unique
   // list (usually all of it) into pinned host memory itself.
   prefix_len_ = std::min(n + 1, kPrefixInts);
-  WalkBitmapsKernel<<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
-                                                   static_cast<const int4*>(partials_), premarked ? 0 : grid, words,
-                                                   static_cast<MarkState*>(host_state_),
-                                                   static_cast<int*>(host_prefix_), prefix_len_);
+  // A premarked walk over a few thousand words takes one 256-thread workgroup;
+  // larger walks and the reduction of many partials take 1024 threads.
+  if (premarked && words.x + words.y <= 4096)
+    WalkBitmapsKernel<256><<<dim3(1), dim3(256), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+                                                         static_cast<const int4*>(partials_), 0, words,
+                                                         static_cast<MarkState*>(host_state_),
+                                                         static_cast<int*>(host_prefix_), prefix_len_);
+  else
+    WalkBitmapsKernel<1024><<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+                                                           static_cast<const int4*>(partials_), premarked ? 0 : grid,
+                                                           words, static_cast<MarkState*>(host_state_),
+                                                           static_cast<int*>(host_prefix_), prefix_len_);
   RT_HIP_CHECK(hipGetLastError());
   RT_HIP_CHECK(hipEventRecord(copied_, s));
   return RT_OK;

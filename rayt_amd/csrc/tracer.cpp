@@ -100,7 +100,7 @@ int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premar
 }
 
 // Resident frames between rebuilds of the tile order: the rebuild (one
-// workgroup, about 20 us) follows the first frame of a launch shape and then
+// workgroup, about 30 us) follows the first frame of a launch shape and then
 // every kReorderFrames-th frame; the frames between keep the order.
 constexpr uint64_t kReorderFrames = 8;
 
