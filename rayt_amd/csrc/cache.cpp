@@ -2,7 +2,10 @@
 #include "cache.h"
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <queue>
 #include <stdexcept>
 #include <thread>
@@ -58,6 +61,7 @@ BlockPool::BlockPool(rt_ctx* ctx, const StoredOctreeHeader& header, int max_bloc
 }
 
 BlockPool::~BlockPool() {
+  workers_.reset();
   hipStreamSynchronize(ctx_->stream);
   for (rt_buf* b : channels_) rt_buf_destroy(b);
   rt_buf_destroy(far_);
@@ -98,24 +102,118 @@ void BlockPool::StageBlockCopy(int slot, uint32_t tree_block, const uint8_t* src
     memcpy(dst, src, content_);
 }
 
-// The payload copies of a batch: a few MB per transaction out of core, spread
-// over up to 8 threads (one thread reaches a fraction of the host's memory
-// bandwidth).
-void BlockPool::CopyPending() {
-  const size_t n = pending_copies_.size();
-  const int threads = int(std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()), n / 4}));
-  auto run = [&](size_t b, size_t e) {
-    for (size_t i = b; i < e; ++i) memcpy(pending_copies_[i].first, pending_copies_[i].second, content_);
-  };
-  if (threads <= 1) {
-    run(0, n);
-  } else {
-    std::vector<std::thread> pool;
-    for (int t = 1; t < threads; ++t) pool.emplace_back(run, n * t / threads, n * (t + 1) / threads);
-    run(0, n / threads);
-    for (std::thread& th : pool) th.join();
+// Copy threads shared by the flushes of one pool. A job is the list of a
+// batch's pending payload copies; every participant takes the next copy with
+// one atomic, so copies finish roughly in staging order, and counts the
+// copies done per chunk.
+struct BlockPool::CopyWorkers {
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv;
+  uint64_t generation = 0;
+  bool stop = false;
+  // the current job
+  const std::pair<uint8_t*, const uint8_t*>* items = nullptr;
+  size_t n = 0;
+  uint64_t content = 0;
+  const uint8_t* base = nullptr;  // host staging: an item's chunk is (dst - base) / (content * kChunkBlocks)
+  std::vector<std::atomic<int>> chunk_done;
+  std::atomic<size_t> next{0};
+  std::atomic<int> active{0};  // workers inside the job
+
+  explicit CopyWorkers(int count) {
+    for (int i = 0; i < count; ++i) threads.emplace_back([this] { Run(); });
   }
+  ~CopyWorkers() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (std::thread& t : threads) t.join();
+  }
+  int ChunkOf(size_t i) const { return int(uint64_t(items[i].first - base) / (content * kChunkBlocks)); }
+  // Copies until the job runs out; returns after its last copy.
+  void Work() {
+    for (size_t i; (i = next.fetch_add(1)) < n;) {
+      memcpy(items[i].first, items[i].second, content);
+      chunk_done[ChunkOf(i)].fetch_add(1, std::memory_order_release);
+    }
+  }
+  void Run() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || generation != seen; });
+        if (stop) return;
+        seen = generation;
+      }
+      Work();
+      active.fetch_sub(1, std::memory_order_acq_rel);
+    }
+  }
+};
+
+// The payload copies of a batch (a few MB per transaction out of core) and
+// the transfers of the payload area [0, nb * content) to the device staging
+// buffer, chunk by chunk as the chunks are filled.
+int BlockPool::CopyAndTransferPayloads(int nb) {
+  const size_t n = pending_copies_.size();
+  const int nchunks = (nb + kChunkBlocks - 1) / kChunkBlocks;
+  auto transfer = [&](int c) -> int {
+    const uint64_t b = uint64_t(c) * kChunkBlocks * content_;
+    const uint64_t e = std::min<uint64_t>(uint64_t(nb) * content_, b + uint64_t(kChunkBlocks) * content_);
+    RT_HIP_CHECK(hipMemcpyAsync(dev_stage_ + b, host_stage_ + b, e - b, hipMemcpyHostToDevice, ctx_->stream));
+    return RT_OK;
+  };
+  if (n < 8) {  // a small batch: copy here, one transfer
+    for (auto& cp : pending_copies_) memcpy(cp.first, cp.second, content_);
+    pending_copies_.clear();
+    if (nb == 0) return RT_OK;
+    RT_HIP_CHECK(hipMemcpyAsync(dev_stage_, host_stage_, uint64_t(nb) * content_, hipMemcpyHostToDevice,
+                                ctx_->stream));
+    return RT_OK;
+  }
+  if (!workers_) {
+    const int hw = int(std::max(2u, std::thread::hardware_concurrency()));
+    workers_.reset(new CopyWorkers(std::min(7, hw - 1)));
+  }
+  CopyWorkers& w = *workers_;
+  std::vector<int> expected(size_t(nchunks), 0);
+  {
+    std::lock_guard<std::mutex> lk(w.mu);
+    w.items = pending_copies_.data();
+    w.n = n;
+    w.content = content_;
+    w.base = host_stage_;
+    std::vector<std::atomic<int>> counts(static_cast<size_t>(nchunks));
+    for (auto& a : counts) a.store(0, std::memory_order_relaxed);
+    w.chunk_done.swap(counts);
+    for (size_t i = 0; i < n; ++i) ++expected[size_t(w.ChunkOf(i))];
+    w.next.store(0);
+    w.active.store(int(w.threads.size()));
+    ++w.generation;
+  }
+  w.cv.notify_all();
+  int issued = 0, rc = RT_OK;
+  auto issue_ready = [&] {
+    while (rc == RT_OK && issued < nchunks &&
+           w.chunk_done[size_t(issued)].load(std::memory_order_acquire) == expected[size_t(issued)])
+      rc = transfer(issued++);
+  };
+  for (size_t i; (i = w.next.fetch_add(1)) < n;) {
+    memcpy(w.items[i].first, w.items[i].second, content_);
+    w.chunk_done[size_t(w.ChunkOf(i))].fetch_add(1, std::memory_order_release);
+    issue_ready();
+  }
+  while (w.active.load(std::memory_order_acquire) != 0) {  // the workers' last copies
+    issue_ready();
+    std::this_thread::yield();
+  }
+  issue_ready();
   pending_copies_.clear();
+  return rc != RT_OK ? rc : (issued == nchunks ? RT_OK : Fail(RT_ERR_STATE, "payload chunks not transferred"));
 }
 
 void BlockPool::StageLinkWrite(uint64_t node, uint32_t value) {
@@ -163,7 +261,8 @@ int BlockPool::Flush() {
   const uint64_t off_fwv = off;
   off += uint64_t(nfw) * 4;
   if (off > stage_bytes_) return Fail(RT_ERR_STATE, "staging overflow");
-  CopyPending();
+  rc = CopyAndTransferPayloads(nb);
+  if (rc) return rc;
   memcpy(host_stage_ + off_slots, staged_slots_.data(), size_t(nb) * 4);
   memcpy(host_stage_ + off_tb, staged_tree_blocks_.data(), size_t(nb) * 4);
   for (int i = 0; i < nlw; ++i) {
@@ -174,7 +273,8 @@ int BlockPool::Flush() {
     memcpy(host_stage_ + off_fwi + size_t(i) * 4, &far_writes_[i].first, 4);
     memcpy(host_stage_ + off_fwv + size_t(i) * 4, &far_writes_[i].second, 4);
   }
-  RT_HIP_CHECK(hipMemcpyAsync(dev_stage_, host_stage_, off, hipMemcpyHostToDevice, ctx_->stream));
+  RT_HIP_CHECK(hipMemcpyAsync(dev_stage_ + off_slots, host_stage_ + off_slots, off - off_slots,
+                              hipMemcpyHostToDevice, ctx_->stream));
   in_flight_ = true;
   PoolBatchArgs a;
   a.stage = dev_stage_;

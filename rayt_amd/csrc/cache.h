@@ -71,7 +71,14 @@ class BlockPool {
   std::vector<std::pair<uint32_t, uint32_t>> far_writes_;
   std::vector<std::pair<uint8_t*, const uint8_t*>> pending_copies_;  // staging payload <- stable source
   bool in_flight_ = false;
-  void CopyPending();
+  // Persistent copy threads (started by the first large batch): a batch's
+  // payload copies are taken in staging order by them and the flushing thread,
+  // and each chunk of kChunkBlocks payloads is queued for transfer as soon as
+  // it is copied, so the copies overlap the transfer of earlier chunks.
+  static constexpr int kChunkBlocks = 16;
+  struct CopyWorkers;
+  std::unique_ptr<CopyWorkers> workers_;
+  int CopyAndTransferPayloads(int nb);
   int WaitStaging();
 };
 
