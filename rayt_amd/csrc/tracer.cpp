@@ -32,7 +32,8 @@ struct rt_tracer {
   rt_buf* tile_cost = nullptr;
   rt_buf* tile_order = nullptr;
   int sched_waves = 0;
-  uint64_t sched_frames = 0;  // resident frames since the order was last rebuilt
+  bool order_ready = false;   // tile_order holds an order for sched_waves (written by a queued rebuild)
+  uint64_t sched_frames = 0;  // resident frames since the shape's first
   rt_buf* states = nullptr;
   rt_feedback* feedback = nullptr;
   uint64_t stats[5] = {0, 0, 0, 0, 0};
@@ -113,8 +114,11 @@ rayt::TileSchedule Schedule(rt_tracer* t) {
   rayt::PlainTiles(t->ctx, t->width, t->height, &tiles, &waves);
   s.order = static_cast<uint32_t*>(t->tile_order->ptr);
   s.cost = static_cast<uint32_t*>(t->tile_cost->ptr);
-  s.order_valid = t->sched_waves == waves;
-  if (!s.order_valid) t->sched_frames = 0;
+  if (t->sched_waves != waves) {  // a new launch shape starts again from raster order
+    t->order_ready = false;
+    t->sched_frames = 0;
+  }
+  s.order_valid = t->order_ready;
   t->sched_waves = waves;
   if (tiles > uint32_t(rayt::kMaxScheduledTiles)) {  // raster order, no costs: nothing to order
     s.order_valid = false;
@@ -129,7 +133,10 @@ int MaybeReorder(rt_tracer* t, const rayt::TileSchedule& sched) {
   if (!sched.cost) return RT_OK;
   const bool due = t->sched_frames % kReorderFrames == 0;
   ++t->sched_frames;
-  return due ? rayt::OrderTiles(t->ctx, sched, t->width, t->height) : RT_OK;
+  if (!due) return RT_OK;
+  const int rc = rayt::OrderTiles(t->ctx, sched, t->width, t->height);
+  if (rc == RT_OK) t->order_ready = true;
+  return rc;
 }
 
 hipEvent_t PipelineEvent(rt_tracer* t, int buf, int which) {
