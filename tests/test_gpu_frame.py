@@ -31,6 +31,7 @@ def _run(gpu_ctx, depth, nib, boxes, pool_frac, W, H, flags, max_passes, lod, po
     assert m.root_node_index() == c.root()
     rng, ef = 99, 1 << 16
     total_uploads = 0
+    most_in_frame = 0
     for k, (yaw, pitch, pos) in enumerate(poses):
         vpi = camera_vpi(yaw, pitch, pos, aspect=W / H)
         tr.RenderFrame(vpi)
@@ -46,14 +47,17 @@ def _run(gpu_ctx, depth, nib, boxes, pool_frac, W, H, flags, max_passes, lod, po
         np.testing.assert_array_equal(pool["far"], c.far())
         np.testing.assert_array_equal(pool["normals"], c.channel_bytes(1, 4))
         assert m.lru_order() == c.lru_order(), f"LRU order differs after frame {k}"
+        most_in_frame = max([most_in_frame] + [len(p) for p in tr.pass_faults()])
         total_uploads = m.counters()[0]
-    return total_uploads, m.counters()
+    return total_uploads, m.counters(), most_in_frame
 
 
 def test_frame_out_of_core_small_pool(gpu_ctx):
-    uploads, counters = _run(gpu_ctx, depth=9, nib=512, boxes=16, pool_frac=0.35, W=320, H=240, flags=0,
-                             max_passes=6, lod=0.0, poses=POSES[:4])
+    uploads, counters, most = _run(gpu_ctx, depth=9, nib=512, boxes=16, pool_frac=0.35, W=320, H=240, flags=0,
+                                   max_passes=6, lod=0.0, poses=POSES[:4])
     assert counters[1] > 0, "the pool was expected to evict blocks"
+    # a transaction that requests blocks over several 16-block staging chunks (copy threads, chunked transfers)
+    assert most >= 32, most
 
 
 def test_frame_out_of_core_shadows_lod(gpu_ctx):
