@@ -67,3 +67,20 @@ def test_image_dumps_round_trip(tmp_path):
     codes = np.array([[-1, 0x7FFFFFFF, (3 << 1) | 1, 3 << 1]], np.int32)
     cc = imageio.code_colors(codes)
     assert cc[0, 0] == 0xFF0000FF and cc[0, 1] == 0xFF000000 and cc[0, 2] != cc[0, 3]
+
+
+def test_missing_library_fails_loudly():
+    """Without the HIP library every entry point raises; nothing falls back to another implementation."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("from rayt_amd import api\n"
+            "try:\n"
+            "    api.Context()\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n")
+    env = dict(os.environ, RAYT_HIP_LIB=os.path.join(root, "_no_such_dir", "librayt_hip.so"))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "raised:" in r.stdout and "is not built" in r.stdout
