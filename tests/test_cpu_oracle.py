@@ -11,7 +11,11 @@ application/test-application.cpp:134-291), so the oracle is pinned by:
     restated) on pools filled by the oracle's cache, with a planted mistake
     that it must catch;
   * LRU invariants of gpu-octree-cache.h:106 (a child is always nearer the
-    least-recently-used end than its parent) and load/unload round trips.
+    least-recently-used end than its parent), and after every transaction of
+    loads and evictions the pool words and far pointers implied by the set of
+    loaded blocks, derived from the .tree file alone;
+  * a closed-form shadow scene: exactly the floor pixels whose ray towards
+    light 1 crosses an occluder lose light 1's term.
 """
 import math
 
@@ -180,22 +184,96 @@ def test_shading_closed_form():
     assert (bg == np.array([0, 216, 255, 255], np.uint8)).all()
 
 
-def test_shadow_ray_blocks_light_one():
-    """A floor lit by light 1 under an overhang: shadowed pixels lose exactly light 1's term."""
-    _, path = terrain_tree(6, 512, 0, 6)
-    t, c = _oracle_resident(path)
-    links, far = c.links(), c.far()
-    normals = c.channel_bytes(1, 4).view(np.uint32)
-    colors = c.channel_bytes(2, 4).view(np.uint32)
-    vpi = camera_vpi(*POSES[0])
-    a = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, 96, 72, vpi, flags=0)
-    b = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, 96, 72, vpi, flags=1)
-    np.testing.assert_array_equal(a["codes"], b["codes"])  # shadows never change what is hit
-    da = a["image"].view(np.uint8).reshape(-1, 4)[:, :3].astype(int)
-    db = b["image"].view(np.uint8).reshape(-1, 4)[:, :3].astype(int)
-    assert (db <= da).all()  # occlusion only removes light
-    assert (db < da).any() and (db == da).any()
-    assert b["stats"][1] > 0
+def _voxel_pool(depth, cells):
+    """Cache-format pool (one block, near pointers) holding the leaf cells {(i, j, k): (normal, rgb)} of a
+    2^depth grid; inner nodes get a zero normal and a white colour. Nodes are laid out breadth first with
+    the children of a node contiguous, root first."""
+    nodes, level = [(0, (0, 0, 0))], [(0, 0, 0)]  # (depth, cell)
+    for d in range(1, depth + 1):
+        present = {(i >> (depth - d), j >> (depth - d), k >> (depth - d)) for i, j, k in cells}
+        level = [c for i, j, k in level for c in ((2 * i + (ch & 1), 2 * j + ((ch >> 1) & 1),
+                                                   2 * k + ((ch >> 2) & 1)) for ch in range(8)) if c in present]
+        nodes += [(d, c) for c in level]
+    index = {n: x for x, n in enumerate(nodes)}
+    links = np.zeros(len(nodes), np.uint32)
+    normals = np.zeros(len(nodes), np.uint32)
+    colors = np.full(len(nodes), 0xFFFFFFFF, np.uint32)
+    for x, (d, (i, j, k)) in enumerate(nodes):
+        if d == depth:
+            n, rgb = cells[(i, j, k)]
+            normals[x] = np.frombuffer(np.array(n + (0,), np.int8).tobytes(), np.uint32)[0]
+            colors[x] = np.frombuffer(np.array(rgb + (255,), np.uint8).tobytes(), np.uint32)[0]
+            continue
+        kids = [(ch, (2 * i + (ch & 1), 2 * j + ((ch >> 1) & 1), 2 * k + ((ch >> 2) & 1))) for ch in range(8)]
+        kids = [(ch, c) for ch, c in kids if (d + 1, c) in index]
+        first = index[(d + 1, kids[0][1])]
+        # children in child-index order occupy consecutive slots (the popcount addressing of raycast.cl:151)
+        assert [index[(d + 1, c)] for _, c in kids] == list(range(first, first + len(kids)))
+        links[x] = (((1 << 20) + first - x) << 11) | sum(1 << ch for ch, _ in kids)
+    return links, np.zeros(8, np.uint32), normals, colors
+
+
+def _crosses_box(q, s, lo, hi):
+    """Whether the ray q + u s (u > 0) passes through the box [lo, hi], in double precision."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0, t1 = (lo - q) / s, (hi - q) / s
+    return np.nanmin(np.maximum(t0, t1)) > max(np.nanmax(np.minimum(t0, t1)), 0.0)
+
+
+def test_shadow_ray_closed_form():
+    """A floor lit by light 1 and a slab hovering above it, seen from straight above. Exactly the floor
+    pixels whose ray towards light 1 (-normalize(2,-2,3), shading.cl:8) crosses the slab, found here in
+    double precision, lose light 1's term; every other floor pixel keeps the unshadowed colour. The shadow
+    ray starts a quarter of the hit voxel (2^-(depth+2)) back along the primary ray, as the extension
+    defines it."""
+    D, G = 4, 16
+    floor_rgb, slab_rgb, up = (200, 150, 100), (90, 90, 90), (0, 127, 0)
+    cells = {(i, 0, k): (up, floor_rgb) for i in range(G) for k in range(G)}
+    slab = {(i, 3, k): (up, slab_rgb) for i in (4, 5, 6) for k in (3, 4)}
+    cells.update(slab)
+    links, far, normals, colors = _voxel_pool(D, cells)
+    floor_ids = {x for x in range(len(links)) if links[x] == 0 and (colors[x] & 0xFF) == floor_rgb[0]}
+    W, H = 128, 128
+    vpi = camera_vpi(0.0, -90.0, (0.5, 2.0, 0.5), fov=28.0, aspect=1.0)
+    lit = orc.render_plain(links, far, 0, normals, colors, 0, W, H, vpi, flags=0, want_hits=True)
+    sh = orc.render_plain(links, far, 0, normals, colors, 0, W, H, vpi, flags=1, want_hits=True)
+    np.testing.assert_array_equal(lit["codes"], sh["codes"])  # shadows never change what is hit
+    # every hit faces up, so light 1 reaches it unless occluded: one shadow ray per hit
+    assert sh["stats"][1] == int((lit["hit_node"] >= 0).sum()) and lit["stats"][1] == 0
+
+    # closed-form ProcessHit colours of an up-facing normal: light 1 adds 1.2/sqrt(17), light 2 1.5/sqrt(22)
+    l1, l2 = 0.6 * 2 / math.sqrt(17), 0.5 * 3 / math.sqrt(22)
+    full = lambda rgb: np.array([min(255.0, ch * (0.1 + l1 + l2)) for ch in rgb])  # noqa: E731
+    dark = lambda rgb: np.array([min(255.0, ch * (0.1 + l2)) for ch in rgb])  # noqa: E731
+    s_dir = -np.array([2.0, -2.0, 3.0]) / math.sqrt(17)
+    lo, hi = np.array([4, 3, 3]) / G, np.array([7, 4, 5]) / G
+    px_lit = lit["image"].view(np.uint8).reshape(H, W, 4)[..., :3].astype(float)
+    px_sh = sh["image"].view(np.uint8).reshape(H, W, 4)[..., :3].astype(float)
+    shadowed = unshadowed = 0
+    for y in range(H):
+        for x in range(W):
+            node = int(lit["hit_node"][y, x])
+            if node < 0:  # background
+                np.testing.assert_array_equal(px_sh[y, x], px_lit[y, x])
+                continue
+            if node not in floor_ids:  # the slab: lit, or shadowed by its own voxels on its far sides
+                assert np.all(np.abs(px_lit[y, x] - full(slab_rgb)) <= 1.0)
+                assert np.array_equal(px_sh[y, x], px_lit[y, x]) or np.all(np.abs(px_sh[y, x] - dark(slab_rgb)) <= 1)
+                continue
+            assert np.all(np.abs(px_lit[y, x] - full(floor_rgb)) <= 1.0), (x, y)
+            o, d = _ray(vpi, x, y, W, H)
+            t = (1.0 / G - o[1]) / d[1]  # the floor's top face
+            q = o + d * (t - 2.0 ** -(D + 2))
+            inside = [_crosses_box(q, s_dir, lo + e, hi - e) for e in (1e-4, -1e-4)]
+            if inside[0] != inside[1]:
+                continue  # grazes the slab's boundary: not decided at this precision
+            if inside[0]:
+                shadowed += 1
+                assert np.all(np.abs(px_sh[y, x] - dark(floor_rgb)) <= 1.0), (x, y)
+            else:
+                unshadowed += 1
+                np.testing.assert_array_equal(px_sh[y, x], px_lit[y, x], err_msg=str((x, y)))
+    assert shadowed > 500 and unshadowed > 10000
 
 
 # --------------------------------------------------------------- camera
@@ -297,31 +375,148 @@ def test_initial_fill_is_bfs_and_lru_invariant():
             assert pos[s] < pos[slot_of[p]], "a child must be nearer the LRU end than its parent"
 
 
-def test_load_unload_restores_parent_links():
-    """Loading a block rewrites its parent's pointers; evicting it restores the stored-format words."""
-    _, path = terrain_tree(7, 128, 0, 4)
-    t = orc.Tree(path)
-    c = orc.Cache(t, 3)
-    c.initial_fill()  # root + its first two BFS children
-    before = c.links().copy()
+ROOT_BLOCK_PARENT = (1 << 24) - 1
+
+
+class _Blocks:
+    """Headers and stored links of every block of a .tree file, read once."""
+
+    def __init__(self, t):
+        self.t, self.nib, self._d = t, t.nodes_in_block, {}
+
+    def __getitem__(self, b):
+        if b not in self._d:
+            hdr, data = self.t.load_block(b)
+            self._d[b] = (hdr, data[:4 * self.nib].view(np.uint32).copy())
+        return self._d[b]
+
+
+def _pack_cache(mask, fault, dup, far, ptr):
+    """PackCacheNodeLink (octree-constants.h:41-44)."""
+    return (int(ptr) << 11) | (int(far) << 10) | (int(dup) << 9) | (int(fault) << 8) | int(mask)
+
+
+def _expected_pool(blocks, c):
+    """The pool words and far pointers implied by the set of loaded blocks and their slots, derived from the
+    .tree file alone: a block's own non-fault links become near pointers relative to the node holding them,
+    its fault links stay as stored, and each root of a loaded child block turns its parent's pointer into a
+    near or far pointer to the root's slot, flagged duplicate when the stored parent pointer was not a fault
+    (gpu-octree-cache.cpp:119-141, 169-208). Unloading must restore exactly this state for the parent."""
+    nib = blocks.nib
+    slot_of = {b: s for b, s in ((b, c.block_slot(b)) for b in range(blocks.t.blocks)) if s >= 0}
+    links = {}
+    for b, s in slot_of.items():
+        L = blocks[b][1].astype(np.int64)
+        mask, fault, dup, ptr = L & 255, (L >> 8) & 1, (L >> 9) & 1, L >> 11
+        n = np.arange(nib)
+        near = (((1 << 20) + ptr - n) << 11) | (dup << 9) | mask
+        links[s] = np.where((mask != 0) & (fault == 0), near, L).astype(np.uint32)
+    far = {}
+    for b, s in slot_of.items():
+        hdr = blocks[b][0]
+        if int(hdr[1]) == ROOT_BLOCK_PARENT:
+            continue
+        sp = slot_of[int(hdr[1])]  # a loaded block's parent is loaded
+        for i in range(int(hdr[2])):
+            pidx, cidx, pv = int(hdr[3 + 3 * i]), int(hdr[4 + 3 * i]), int(hdr[5 + 3 * i])
+            target, at = s * nib + cidx, sp * nib + pidx
+            dup = 1 - ((pv >> 8) & 1)
+            if abs(at - target) >= 1 << 20:
+                far[s * 8 + i] = target
+                links[sp][pidx] = _pack_cache(pv & 255, 0, dup, 1, s * 8 + i)
+            else:
+                links[sp][pidx] = _pack_cache(pv & 255, 0, dup, 0, (1 << 20) + target - at)
+    return slot_of, links, far
+
+
+def _assert_pool_state(blocks, c):
+    slot_of, exp_links, exp_far = _expected_pool(blocks, c)
+    nib = blocks.nib
+    links, far = c.links(), c.far()
+    for s, words in exp_links.items():
+        got = links[s * nib:(s + 1) * nib]
+        bad = np.nonzero(got != words)[0]
+        assert len(bad) == 0, f"slot {s}: {len(bad)} words differ, first at node {bad[0]}: " \
+                              f"{int(got[bad[0]]):#x} != {int(words[bad[0]]):#x}"
+    for i, v in exp_far.items():
+        assert int(far[i]) == v, f"far pointer {i}"
+    return slot_of, exp_far
+
+
+def _requestable(blocks, loaded, parents=None):
+    """Blocks not loaded that a fault link of a loaded block (of `parents`, when given) points to."""
+    out = set()
+    for b in (loaded if parents is None else parents):
+        L = blocks[b][1]
+        out.update(int(v >> 11) for v in L[((L & 255) != 0) & (((L >> 8) & 1) == 1)])
+    return sorted(out - set(loaded))
+
+
+def _transaction(c, blocks):
     c.StartRequestSession()
-    # request a child of slot 0 that is not loaded, forcing an eviction
-    _, data = t.load_block(t.root_block)
-    lk = data[:4 * t.nodes_in_block].view(np.uint32)
-    kids = [int(l >> 11) for l in lk if (l & 255) and (l >> 8) & 1]
-    missing = [k for k in kids if c.block_slot(k) < 0]
-    assert missing
     c.StartRequestTransaction()
-    c.RequestBlock(missing[0])
+    for b in blocks:
+        c.RequestBlock(b)
     c.EndRequestTransaction()
     c.EndRequestSession()
-    after = c.links()
-    evicted = [s for s in range(3) if s != 0]
-    # Slot 0 (root block) words outside the pointers of the evicted / loaded children are unchanged.
+
+
+def test_load_unload_restores_parent_links():
+    """Loading a block rewrites its parent's pointers to it; evicting it restores the parent's words to
+    their values before the load, bit for bit (gpu-octree-cache.cpp:243-293), and the whole pool stays the
+    state the loaded set implies."""
+    _, path = terrain_tree(7, 128, 0, 4)
+    t = orc.Tree(path)
     nib = t.nodes_in_block
-    changed = np.nonzero(before[:nib] != after[:nib])[0]
-    assert len(changed) > 0
-    assert evicted
+    blocks = _Blocks(t)
+    c = orc.Cache(t, 3)
+    c.initial_fill()  # root + its first two BFS children
+    loaded0, _ = _assert_pool_state(blocks, c)
+    root_slot = loaded0[t.root_block]
+    before = c.links().copy()
+    target = _requestable(blocks, loaded0, [t.root_block])[0]
+    hdr = blocks[target][0]
+    assert int(hdr[1]) == t.root_block
+    ptr_words = [root_slot * nib + int(hdr[3 + 3 * i]) for i in range(int(hdr[2]))]
+    _transaction(c, [target])
+    loaded, _ = _assert_pool_state(blocks, c)
+    assert target in loaded and len(set(loaded0) - set(loaded)) == 1  # one block was evicted for it
+    mid = c.links()
+    assert all(mid[w] != before[w] for w in ptr_words)  # the parent now points at the loaded block
+    # load the root's other children one at a time until the target is evicted in turn
+    for b in _requestable(blocks, loaded0, [t.root_block]):
+        if target not in loaded:
+            break
+        if b not in loaded:
+            _transaction(c, [b])
+            loaded, _ = _assert_pool_state(blocks, c)
+    assert target not in loaded
+    after = c.links()
+    for w in ptr_words:  # the target's parent pointers are back to their words before the load
+        assert after[w] == before[w], f"word {w}: {int(after[w]):#x} != {int(before[w]):#x}"
+    assert c.counters()[1] >= 2  # evictions
+
+
+def test_pool_state_follows_loads_and_evictions_with_far_pointers():
+    """Random request transactions on a pool whose slots lie more than 2^20 nodes apart (far pointers):
+    after every transaction each word of every loaded slot and every live far pointer is the one the set of
+    loaded blocks implies."""
+    _, path = terrain_tree(10, 8192, 0, 4)
+    t = orc.Tree(path)
+    blocks = _Blocks(t)
+    c = orc.Cache(t, 300)
+    c.initial_fill()
+    loaded, far = _assert_pool_state(blocks, c)
+    rng = np.random.default_rng(3)
+    saw_far = len(far)
+    for _ in range(12):
+        cand = _requestable(blocks, loaded)
+        pick = [int(b) for b in rng.choice(cand, size=min(len(cand), int(rng.integers(1, 40))), replace=False)]
+        _transaction(c, pick)
+        loaded, far = _assert_pool_state(blocks, c)
+        saw_far = max(saw_far, len(far))
+    uploaded, evicted = c.counters()[:2]
+    assert evicted > 50 and saw_far > 0
 
 
 # ------------------------------------------------------------- feedback
