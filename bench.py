@@ -45,33 +45,13 @@ def measured_traffic():
     return None
 
 
-MIX_SIZES = [(256, 256), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
-MIX_WEIGHTS = [0.3, 0.3, 0.2, 0.15, 0.05]
-
-
-def view_mix(n: int, seed: int):
-    """n views: seeded resolutions from the mix, positions uniform in [0,1]^3, directions uniform on the sphere."""
-    import math
-    import numpy as np
-    from rayt_amd import api
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(n):
-        w, h = MIX_SIZES[rng.choice(len(MIX_SIZES), p=MIX_WEIGHTS)]
-        pos = tuple(float(x) for x in rng.uniform(0.0, 1.0, 3))
-        yaw = float(rng.uniform(-180.0, 180.0))
-        pitch = math.degrees(math.asin(float(rng.uniform(-1.0, 1.0))))
-        vpi = api.Camera(yaw=yaw, pitch=pitch, position=pos, fov=60.0, aspect=w / h).view_proj_inv()
-        specs.append((vpi, w, h))
-    return specs
-
-
 def measure_batched(ctx, cache, depth: int, steps: int, nviews: int = 64, seed: int = 7, shape=None):
     """64-view mix in one batched launch vs the plain kernel launched once per view.
     shape: (tile, LDS levels, workgroups per CU) instead of the autotuner's choice."""
     import numpy as np
     from rayt_amd import api
     flags = api.L.RT_FLAG_SHADOWS
+    from rayt_amd.workloads import view_mix
     specs = view_mix(nviews, seed)
     b = api.BatchRenderer(cache, nviews)
     b.config(tile_shape=0, lds_levels=12, wgs_per_cu=16, tree_depth=depth)
@@ -155,6 +135,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     max_block_loads bounds the blocks loaded per out-of-core scene between rounds."""
     import numpy as np
     from rayt_amd import api
+    from rayt_amd.workloads import view_mix
     flags = api.L.RT_FLAG_SHADOWS
     rng = np.random.default_rng(seed)
     budget = int(pool_gb * (1 << 30))
@@ -274,34 +255,6 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
             "burst": closed}
 
 
-def spline_poses(n: int, seed: int):
-    """n camera poses along a closed Catmull-Rom spline through 8 seeded control points above the terrain,
-    each looking at a point of the terrain near the centre that moves with the path (camera convention of
-    camera.cpp: forward = (sin yaw cos pitch, sin pitch, -cos yaw cos pitch))."""
-    import math
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    ctrl = np.column_stack([rng.uniform(0.1, 0.9, 8), rng.uniform(0.55, 0.85, 8), rng.uniform(0.1, 0.9, 8)])
-
-    def point(s):
-        k = int(math.floor(s)) % 8
-        u = s - math.floor(s)
-        p0, p1, p2, p3 = (ctrl[(k + j - 1) % 8] for j in range(4))
-        return 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u * u
-                      + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
-
-    poses = []
-    for i in range(n):
-        s = 8.0 * i / n
-        p, q = point(s), point(s + 0.02)
-        target = np.array([0.5, 0.25, 0.5]) + 2.0 * (q - p)
-        v = target - p
-        yaw = math.degrees(math.atan2(v[0], -v[2]))
-        pitch = math.degrees(math.atan2(v[1], math.hypot(v[0], v[2])))
-        poses.append((yaw, pitch, tuple(float(x) for x in p)))
-    return poses
-
-
 def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, lod: float = 1.0,
                         pool_gb: float = 0.0):
     """Fly-through over a block pool that holds `frac` of the tree's blocks (or `pool_gb` GiB of blocks when
@@ -326,6 +279,7 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     tr.set_lod_voxel_size(lod)
     tr.set_flags(api.L.RT_FLAG_SHADOWS)
     tr.set_seed(3)
+    from rayt_amd.workloads import spline_poses
     poses = spline_poses(frames, seed=11)
     passes = uploads = rays = 0
     ctx.WaitForAll()

@@ -46,6 +46,7 @@ _SIG = {
     "orc_init_tracing_frame": (I, [FP, I, I, FP, F]),
     "orc_raytracing_pass": (I, [I32P, FP, I, I, U32P, U32P, I, U32P, U32P, I, I, I]),
     "orc_finish_tracing_frame": (I, [U32P, FP, I, I, FP, U32P, U32P]),
+    "orc_ln_fixed": (None, [FP, FP, C.c_int64]),
     "orc_raytrace_depth": (I, [U32P, U32P, I, I, I, FP, F, U32P, I32P, I]),
     "orc_raytrace_restart": (I, [U32P, I, I, I, FP, F, U32P, I32P, I]),
     "orc_legacy_links": (I, [U32P, U32P, C.c_uint64, U32P]),
@@ -261,6 +262,14 @@ def raytrace_depth(links, far, root, width, height, vpi, lod=0.0, threads=0):
     lib().orc_raytrace_depth(_p(links, U32P), _p(far, U32P), root, width, height, _p(m, FP), lod, _p(out, U32P),
                              _p(hn, I32P), threads)
     return out, hn
+
+
+def ln_fixed(x):
+    """The fixed-order natural logarithm of the log-depth grey (float32 in, float32 out)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_ln_fixed(_p(x, FP), _p(out, FP), x.size)
+    return out
 
 
 def raytrace_restart(legacy_links, root, width, height, vpi, lod=0.0, threads=0):

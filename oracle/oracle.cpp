@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <list>
 #include <map>
 #include <memory>
@@ -987,9 +988,37 @@ static NearFar NearPlaneRay(const float* m, int x, int y, int W, int H, float lo
   return r;
 }
 
+// ln t in double with a fixed operation order, rounded to float: the
+// reduction t = m * 2^e, m in [sqrt(1/2), sqrt(2)), and the series
+// ln m = 2 atanh(s) = 2 (s + s^3/3 + ... + s^23/23), s = (m - 1) / (m + 1)
+// (Abramowitz & Stegun 4.1.27). The fixed order makes host and GPU agree to
+// the bit (a libm log may differ by an ulp); tests pin it against libm.
+static float LnFixed(float t) {
+  if (!(t > 0.0f)) return t == 0.0f ? -std::numeric_limits<float>::infinity() : std::nanf("");
+  if (std::isinf(t)) return t;
+  double dt = double(t);
+  uint64_t bits;
+  memcpy(&bits, &dt, 8);
+  int e = int((bits >> 52) & 0x7ff) - 1023;
+  uint64_t mb = (bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m;
+  memcpy(&m, &mb, 8);
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 23.0;
+  const double k[11] = {1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0,
+                        1.0 / 9.0,  1.0 / 7.0,  1.0 / 5.0,  1.0 / 3.0,  1.0};
+  for (double c : k) p = p * z + c;
+  return float(double(e) * 0.6931471805599453094 + 2.0 * (s * p));
+}
+
 static uint32_t DepthGrey(float t) {
   const float ln_near = -4.60517025f, ln_far = 1.60943794f;
-  float lt = float(log(double(t)));
+  float lt = LnFixed(t);
   float d = (lt - ln_near) / (ln_far - ln_near);
   d = (1.0f - d) * 255.0f;
   uint32_t g = uint32_t(cl_max(0.0f, cl_min(255.0f, d)));
@@ -1164,6 +1193,11 @@ int orc_finish_tracing_frame(uint32_t* out, const float* states, int W, int H, c
     out[i] = FinishPixel(P, s, bg, L);
   }
   return 0;
+}
+
+// LnFixed over an array (pinned by the tests against libm).
+void orc_ln_fixed(const float* in, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) out[i] = LnFixed(in[i]);
 }
 
 // RaytraceKernel of raytracer.cl (log-depth grey). hit_node may be null.

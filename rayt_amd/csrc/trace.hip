@@ -344,8 +344,9 @@ __global__ __launch_bounds__(256) void RaytraceDepthKernel(uint32_t* __restrict_
 
 // ------------------------------------------- legacy restart traversal (volatile)
 __global__ __launch_bounds__(256) void RaytraceRestartKernel(uint32_t* __restrict__ out, int W, int H,
-                                                             const uint32_t* __restrict__ links, uint32_t root,
-                                                             Mat16 m, float lod) {
+                                                             const uint32_t* __restrict__ links,
+                                                             uint32_t node_count, uint32_t root, Mat16 m,
+                                                             float lod) {
   const uint32_t tiles_x = (W + 15) / 16;
   int x, y;
   TilePixel(blockIdx.x, tiles_x, x, y);
@@ -367,7 +368,8 @@ __global__ __launch_bounds__(256) void RaytraceRestartKernel(uint32_t* __restric
   };
   const float bias = lod * dist(p1n, p2n);
   const float coef = (bias < 1e-10f) ? 0.0f : (lod * (dist(p1f, p2f) - dist(p1n, p2n)) / dist(p1n, p1f));
-  RestartResult res = CastRayRestart(links, root, origin.x, origin.y, origin.z, d.x, d.y, d.z, coef, bias);
+  RestartResult res = CastRayRestart(links, node_count, root, origin.x, origin.y, origin.z, d.x, d.y, d.z, coef,
+                                     bias);
   const int64_t i = int64_t(y) * W + x;
   if (res.error_code) {
     out[i] = PackRGBA(255, 0, 0, 255);
@@ -379,8 +381,11 @@ __global__ __launch_bounds__(256) void RaytraceRestartKernel(uint32_t* __restric
   }
 }
 
+// A far link whose index lies outside the far-pointer table raises the
+// overflow flag (the caller reports a range error) instead of reading past it.
 __global__ void LegacyLinksKernel(uint32_t* __restrict__ out, const uint32_t* __restrict__ links,
-                                  const uint32_t* __restrict__ far_ptrs, uint64_t n, int* __restrict__ overflow) {
+                                  const uint32_t* __restrict__ far_ptrs, uint64_t far_count, uint64_t n,
+                                  int* __restrict__ overflow) {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t l = links[i];
@@ -392,7 +397,7 @@ __global__ void LegacyLinksKernel(uint32_t* __restrict__ out, const uint32_t* __
       target = l >> 11;  // fault: child block index
       o = (1u << 8);
     } else if (l & (1u << 10)) {
-      target = far_ptrs[l >> 11];
+      target = (l >> 11) < far_count ? far_ptrs[l >> 11] : (1ull << 23);
     } else {
       target = uint64_t(int64_t(i) + int64_t(l >> 11) - (1 << 20));
     }
@@ -584,7 +589,8 @@ int rt_raytrace_restart(rt_ctx* ctx, rt_buf* out_image, int width, int height, c
   Mat16 m;
   memcpy(m.m, vpi, sizeof(m.m));
   RaytraceRestartKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
-      static_cast<uint32_t*>(out_image->ptr), width, height, U32(legacy_links), uint32_t(root_node_index), m,
+      static_cast<uint32_t*>(out_image->ptr), width, height, U32(legacy_links),
+      uint32_t(std::min<uint64_t>(legacy_links->size / 4, 0xffffffffu)), uint32_t(root_node_index), m,
       lod_voxel_size);
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
@@ -599,13 +605,16 @@ int rt_legacy_links_from_cache(rt_ctx* ctx, rt_buf* legacy_links, const rt_buf* 
   RT_HIP_CHECK(hipMalloc(&d_over, sizeof(int)));
   hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream);
   LegacyLinksKernel<<<dim3(unsigned((node_count + 255) / 256)), dim3(256), 0, ctx->stream>>>(
-      static_cast<uint32_t*>(legacy_links->ptr), U32(node_links), U32(far_pointers), node_count, d_over);
+      static_cast<uint32_t*>(legacy_links->ptr), U32(node_links), U32(far_pointers), far_pointers->size / 4,
+      node_count, d_over);
   int over = 0;
   hipError_t e = hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipFree(d_over);
   if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
-  RT_REQUIRE(!over, RT_ERR_RANGE, "pool needs more than 23-bit node indices for the legacy layout");
+  RT_REQUIRE(!over, RT_ERR_RANGE,
+             "pool needs more than 23-bit node indices for the legacy layout, or a far link lies outside the "
+             "far-pointer table");
   return RT_OK;
 }
 

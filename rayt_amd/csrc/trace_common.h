@@ -12,13 +12,49 @@ rtdev::Lights MakeLights();
 
 namespace rtdev {
 
+// Natural logarithm of a float, evaluated in double with a fixed operation
+// order and rounded to float, so that every IEEE-754 implementation (the
+// oracle on the host, this kernel on gfx950, both without contraction) gives
+// the same bits; a libm `log` differs between the two by an ulp now and then.
+// Method: the argument reduction t = m * 2^e with m in [sqrt(1/2), sqrt(2))
+// and the series ln m = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...),
+// s = (m - 1) / (m + 1) (Abramowitz & Stegun 4.1.27), whose terms up to s^23
+// bring the truncation below 2^-55 relative for |s| <= 0.1716.
+__device__ __forceinline__ float LnFixed(float t) {
+  if (!(t > 0.0f)) return t == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+  if (t == __builtin_inff()) return t;
+  const uint64_t bits = uint64_t(__double_as_longlong(double(t)));
+  int e = int((bits >> 52) & 0x7ff) - 1023;  // t is a float, so its double is normal
+  double m = __longlong_as_double(int64_t((bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e = e + 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 1.0 / 23.0;
+  p = p * z + 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  p = p * z + 1.0;
+  const double ln2 = 0.6931471805599453094;
+  return float(double(e) * ln2 + 2.0 * (s * p));
+}
+
 // Grey = (1 - (ln t - ln 0.01) / (ln 5 - ln 0.01)) * 255 clamped to [0,255]
-// and truncated. ln is evaluated in double and rounded to float; the two
-// constant logarithms are fixed fp32 values.
+// and truncated (raytracer.cl:63-68), ln from LnFixed; the two constant
+// logarithms are fixed fp32 values.
 __device__ __forceinline__ uint32_t DepthGrey(float t) {
   const float ln_near = -4.60517025f;  // (float)log(0.01)
   const float ln_far = 1.60943794f;    // (float)log(5.0)
-  const float lt = float(log(double(t)));
+  const float lt = LnFixed(t);
   float d = (lt - ln_near) / (ln_far - ln_near);
   d = (1.0f - d) * 255.0f;
   const float g = cl_max(0.0f, cl_min(255.0f, d));
@@ -36,10 +72,13 @@ struct RestartResult {
 __device__ __forceinline__ float MinComponent(float x, float y, float z) { return cl_fmin(cl_fmin(x, y), z); }
 __device__ __forceinline__ float MaxComponent(float x, float y, float z) { return cl_fmax(cl_fmax(x, y), z); }
 
-// Restart ("kd-restart") traversal in the cube [0,1]^3.
-__device__ __forceinline__ RestartResult CastRayRestart(const uint32_t* __restrict__ links, uint32_t root, float ox,
-                                                        float oy, float oz, float dx, float dy, float dz,
-                                                        float ray_size_coef, float ray_size_bias) {
+// Restart ("kd-restart") traversal in the cube [0,1]^3. A node index at or
+// beyond `node_count` (a malformed link buffer) ends the ray with error 2
+// instead of reading outside the buffer.
+__device__ __forceinline__ RestartResult CastRayRestart(const uint32_t* __restrict__ links, uint32_t node_count,
+                                                        uint32_t root, float ox, float oy, float oz, float dx,
+                                                        float dy, float dz, float ray_size_coef,
+                                                        float ray_size_bias) {
   RestartResult r;
   r.fault_block = -1;
   r.error_code = 0;
@@ -71,6 +110,10 @@ __device__ __forceinline__ RestartResult CastRayRestart(const uint32_t* __restri
       if (size < ray_size) {
         r.hit_t = t_min;
         r.hit_node = int(node);
+        return r;
+      }
+      if (node >= node_count) {
+        r.error_code = 2;
         return r;
       }
       const uint32_t link = links[node];

@@ -94,6 +94,22 @@ def _agreement(hit_nodes, bf):
     return agree, len(checked)
 
 
+def test_ln_fixed_matches_libm():
+    """The log-depth grey's fixed-order logarithm against libm's double log rounded to float: equal to the
+    bit on every sample (log-uniform over the float range, uniform over the depth range, powers of two and
+    the reduction's boundary at sqrt(2)), and the special values ln 0 = -inf, ln 1 = 0, ln inf = inf."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(0, 10, 200000), np.exp(rng.uniform(-100, 88, 200000)),
+                        2.0 ** np.arange(-149, 128), [np.sqrt(2.0)]]).astype(np.float32)
+    x = np.concatenate([x, np.nextafter(x, np.float32(np.inf)), np.nextafter(x, np.float32(0))])
+    x = x[(x > 0) & np.isfinite(x)]
+    got = orc.ln_fixed(x)
+    ref = np.log(x.astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    sp = orc.ln_fixed(np.array([0.0, 1.0, np.inf], np.float32))
+    assert sp[0] == -np.inf and sp[1] == 0.0 and sp[2] == np.inf
+
+
 def test_depth_kernel_against_brute_force():
     """Hit voxels equal the brute-force nearest leaf, and the grey level is the log-depth map of raytracer.cl
     applied to the brute-force distance: a least-squares line of grey against ln t must have the map's slope

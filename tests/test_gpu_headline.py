@@ -1,0 +1,134 @@
+"""GPU parity on the measured workloads themselves, against the oracle.
+
+- The single-view scene: depth-11 (2048^3) fBm terrain + 64 boxes, 8192-node
+  blocks, fully resident, 1920x1080, primary rays plus a shadow ray per lit
+  hit, at the measured pose and two others. The plain kernel in the measured
+  launch shape (one wave per workgroup, 10 LDS stack levels, window-only
+  stack) gives the oracle's image, block-level codes and the four counters
+  (primary rays, shadow rays, node fetches F, hits) of
+  raytracing_pass.cl:4-61. GPURayTracer::RenderFrame, pipelined and over
+  scheduled tiles as measured, gives the oracle's frame
+  (gpu-ray-tracer.cpp:66-227): image, codes, fault lists and LRU order, frame
+  after frame.
+- The 64-view mix over the same scene: every view, in both batched kernels
+  (one wave per tile, and persistent waves over a shared tile queue), equals
+  the oracle's render of that view; the mix holds a 3840x2160 view.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from rayt_amd import api
+from rayt_amd.workloads import PLAIN_POSE, camera, plain_scene_spec, view_mix
+from tests.scenes import SCENE_DIR
+
+pytestmark = pytest.mark.gpu
+
+W, H = 1920, 1080
+SHADOWS = api.L.RT_FLAG_SHADOWS
+# the measured pose, a grazing view along the terrain and a steep view from above
+HEADLINE_POSES = [PLAIN_POSE, (120.0, -8.0, (0.9, 0.45, 0.2)), (-60.0, -55.0, (0.7, 1.1, 0.8))]
+ORACLE_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def headline(gpu_ctx):
+    tree = api.StoredOctree.terrain(plain_scene_spec(11), 8192)
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    path = os.path.join(SCENE_DIR, "headline_d11.tree")
+    tree.write(path)
+    info = tree.info()
+    m = api.GPUOctreeCacheManager(info.blocks_count, tree, gpu_ctx)
+    m.InitialFillCache()
+    t = orc.Tree(path)
+    c = orc.Cache(t, t.blocks)
+    c.initial_fill()
+    assert m.root_node_index() == c.root()
+    arrays = (c.links(), c.far(), c.channel_bytes(1, 4).view(np.uint32), c.channel_bytes(2, 4).view(np.uint32))
+    yield tree, path, m, t, c, arrays
+    m.close()
+
+
+def _oracle(headline, vpi, w, h):
+    _, _, _, t, c, (links, far, normals, colors) = headline
+    return orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, w, h, vpi, lod=0.0,
+                            flags=SHADOWS, threads=ORACLE_THREADS)
+
+
+def test_headline_plain_kernel_matches_oracle(gpu_ctx, headline):
+    m = headline[2]
+    api.plain_config(gpu_ctx, 10, 1, 11)
+    try:
+        for yaw, pitch, pos in HEADLINE_POSES:
+            vpi = camera(yaw, pitch, pos, W, H)
+            out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi, lod=0.0, flags=SHADOWS)
+            _, _, st = api.render_plain(gpu_ctx, m, W, H, vpi, lod=0.0, flags=SHADOWS, count=True)
+            r = _oracle(headline, vpi, W, H)
+            pose = f"pose {yaw, pitch, pos}"
+            np.testing.assert_array_equal(codes.read_all(np.int32).reshape(H, W), r["codes"], err_msg=pose)
+            np.testing.assert_array_equal(out.read_all(np.uint32).reshape(H, W), r["image"], err_msg=pose)
+            assert st == r["stats"], pose
+            assert st[0] == W * H and st[1] > 100000 and st[3] > 500000, st
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+
+
+def test_headline_frames_match_oracle(gpu_ctx, headline):
+    """RenderFrame as measured (pipelined, tiles scheduled longest first from earlier frames' wave costs):
+    every frame's image, codes, fault lists and the LRU order equal the oracle's frame loop."""
+    tree, path, _, _, _, _ = headline
+    info = tree.info()
+    m = api.GPUOctreeCacheManager(info.blocks_count, tree, gpu_ctx)
+    m.InitialFillCache()
+    t = orc.Tree(path)
+    c = orc.Cache(t, t.blocks)
+    c.initial_fill()
+    api.plain_config(gpu_ctx, 10, 1, 11)
+    try:
+        tr = api.GPURayTracer(m, W, H)
+        tr.set_frame_time_limit(1e9)
+        tr.set_lod_voxel_size(0.0)
+        tr.set_flags(SHADOWS)
+        tr.set_seed(1)
+        rng, ef = 1, 1 << 16
+        # the measured pose repeated (the tile order takes effect from the second frame), then the others
+        poses = [PLAIN_POSE] * 3 + HEADLINE_POSES[1:] + [PLAIN_POSE]
+        for k, (yaw, pitch, pos) in enumerate(poses):
+            vpi = camera(yaw, pitch, pos, W, H)
+            tr.RenderFrame(vpi)
+            r = c.render_frame(W, H, vpi, lod=0.0, flags=SHADOWS, seed=rng, expected_faults=ef,
+                               threads=ORACLE_THREADS)
+            rng, ef = r["rng"], r["expected_faults"]
+            assert tr.pass_faults() == r["pass_faults"], f"fault lists differ in frame {k}"
+            np.testing.assert_array_equal(tr.codes(), r["codes"], err_msg=f"codes differ in frame {k}")
+            np.testing.assert_array_equal(tr.out_image(), r["image"], err_msg=f"image differs in frame {k}")
+            assert m.lru_order() == c.lru_order(), f"LRU order differs after frame {k}"
+        tr.close()
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+        m.close()
+
+
+@pytest.mark.parametrize("wgs_per_cu", [0, 16])  # one wave per tile (hardware-dispatched); persistent waves
+def test_headline_view_mix_matches_oracle(gpu_ctx, headline, wgs_per_cu):
+    m = headline[2]
+    specs = view_mix(64, 7)
+    assert any((w, h) == (3840, 2160) for _, w, h in specs)
+    b = api.BatchRenderer(m, 64)
+    b.config(tile_shape=1, lds_levels=10, wgs_per_cu=wgs_per_cu, tree_depth=11)
+    views, total, offsets = b.make_views(specs, flags=SHADOWS)
+    out = api.Buffer(gpu_ctx, 4 * total)
+    codes = api.Buffer(gpu_ctx, 4 * total)
+    b.render(views, len(specs), out, codes)
+    img, cod = out.read_all(np.uint32), codes.read_all(np.int32)
+    st = b.render(views, len(specs), out, None, count=True)
+    exp = np.zeros(4, np.int64)
+    for k, ((vpi, w, h), off) in enumerate(zip(specs, offsets)):
+        r = _oracle(headline, vpi, w, h)
+        np.testing.assert_array_equal(cod[off:off + w * h], r["codes"].reshape(-1), err_msg=f"view {k} {w}x{h}")
+        np.testing.assert_array_equal(img[off:off + w * h], r["image"].reshape(-1), err_msg=f"view {k} {w}x{h}")
+        exp += np.array(r["stats"], np.int64)
+    assert st == [int(x) for x in exp]
+    b.close()
