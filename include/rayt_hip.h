@@ -151,6 +151,16 @@ int rt_compress(rt_ctx* ctx, int n, const rt_buf* index, const rt_buf* in, rt_bu
 int rt_sort_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp);
 /* GPUScan::Scan - inclusive prefix sum of int32 in place: GPU algorithms/gpu-scan.cpp:15-43. */
 int rt_scan_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp);
+/* No counterpart in the reference. rt_sort_i32 that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same kernels and
+ * result, with the ping-pong buffer and tile histograms in a static device
+ * array of each GPU (about 36 MiB, held until the process ends). n above
+ * RT_SORT_NOALLOC_MAX_N is refused with RT_ERR_RANGE. Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race; calls queued on one context do not. */
+#define RT_SORT_NOALLOC_MAX_N (1 << 23)
+int rt_sort_i32_noalloc(rt_ctx* ctx, rt_buf* data, int n);
 
 /* --------------------------------------------------------- block sources */
 typedef struct rt_tree_info {

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("RAYT_HIP_LIB") or os.path.join(PKG_DIR, "librayt_hip.
 HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "rayt_hip.h")
 
 RT_OK = 0
+RT_ERR_RANGE = -6
 RT_FLAG_SHADOWS = 1
 RT_FLAG_COUNT = 2
 RT_STATE_BYTES_PER_PIXEL = 48
@@ -76,6 +77,7 @@ SIGNATURES = {
     "rt_compress": (I, [P, I, P, P, P]),
     "rt_sort_i32": (I, [P, P, I, P]),
     "rt_scan_i32": (I, [P, P, I, P]),
+    "rt_sort_i32_noalloc": (I, [P, P, I]),
     "rt_tree_open": (I, [C.c_char_p, PP]),
     "rt_tree_build_sphere": (I, [F, F, F, F, I, I, PP]),
     "rt_tree_build_terrain": (I, [C.POINTER(TerrainParams), I, PP]),

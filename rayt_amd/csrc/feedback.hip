@@ -11,6 +11,7 @@
 // fall back to the radix-sort path.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "block_scan.h"
@@ -416,11 +417,11 @@ int Grid(int64_t n, int block) { return int((n + block - 1) / block); }
 }  // namespace
 
 // ------------------------------------------------------------ host helpers
-int SortI32(hipStream_t s, int* data, int n, int* temp) {
-  if (n <= 1) return RT_OK;
+// The four 8-bit passes of the radix sort, ping-ponging between `data` and
+// `temp` (the result ends in `data`); `hist` holds 256 counters per tile and
+// is written in full by every upsweep, so it needs no clearing.
+static void SortPasses(hipStream_t s, int* data, int n, int* temp, uint32_t* hist) {
   const int tiles = Grid(n, kSortTile);
-  uint32_t* hist = nullptr;
-  RT_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&hist), size_t(tiles) * 256 * 4, s));
   int* a = data;
   int* b = temp;
   for (int shift = 0; shift < 32; shift += 8) {
@@ -429,10 +430,44 @@ int SortI32(hipStream_t s, int* data, int n, int* temp) {
     RadixScatterKernel<<<dim3(tiles), dim3(kSortBlock), 0, s>>>(a, b, n, shift, hist, tiles);
     std::swap(a, b);
   }
-  // four passes: the result is back in `data`
+}
+
+int SortI32(hipStream_t s, int* data, int n, int* temp) {
+  if (n <= 1) return RT_OK;
+  const int tiles = Grid(n, kSortTile);
+  uint32_t* hist = nullptr;
+  RT_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&hist), size_t(tiles) * 256 * 4, s));
+  SortPasses(s, data, n, temp, hist);
   hipError_t e = hipGetLastError();
   (void)hipFreeAsync(hist, s);
   if (e != hipSuccess) return Fail(RT_ERR_HIP, hipGetErrorString(e));
+  return RT_OK;
+}
+
+// Scratch of the sort that allocates nothing (rt_sort_i32_noalloc): the
+// ping-pong buffer of kNoAllocSortMax ints, then the tile histograms. One
+// copy per GPU, for the life of the process. In host code `scratch` names a
+// host-side stand-in, never the device memory: the device address comes from
+// hipGetSymbolAddress, once per device, when a context is created there.
+constexpr int kNoAllocSortMax = RT_SORT_NOALLOC_MAX_N;
+constexpr size_t kNoAllocHistWords = size_t((kNoAllocSortMax + kSortTile - 1) / kSortTile) * 256;
+static __device__ __align__(256) uint32_t scratch[size_t(kNoAllocSortMax) + kNoAllocHistWords];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static uint32_t* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int NoAllocScratchInit(int device) {
+  RT_REQUIRE(device >= 0 && device < kMaxScratchDevices, RT_ERR_RANGE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return RT_OK;
+  int current = -1;
+  RT_HIP_CHECK(hipGetDevice(&current));
+  RT_REQUIRE(current == device, RT_ERR_STATE, "scratch set-up needs its device current");
+  void* ptr = nullptr;
+  RT_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  RT_REQUIRE(ptr, RT_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+  g_scratch_ptr[device] = static_cast<uint32_t*>(ptr);
   return RT_OK;
 }
 
@@ -719,6 +754,25 @@ int rt_sort_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp) {
   RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= data->size && uint64_t(n) * 4 <= temp->size, RT_ERR_RANGE,
              "n exceeds buffer");
   return rayt::SortI32(ctx->stream, static_cast<int*>(data->ptr), n, static_cast<int*>(temp->ptr));
+}
+
+int rt_sort_i32_noalloc(rt_ctx* ctx, rt_buf* data, int n) {
+  RT_REQUIRE(ctx && data, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= data->size, RT_ERR_RANGE, "n exceeds buffer");
+  RT_REQUIRE(n <= rayt::kNoAllocSortMax, RT_ERR_RANGE,
+             "n exceeds the no-allocation sort's maximum of RT_SORT_NOALLOC_MAX_N (8388608) elements");
+  if (n <= 1) return RT_OK;
+  RT_REQUIRE(ctx->device >= 0 && ctx->device < rayt::kMaxScratchDevices, RT_ERR_RANGE, "device index");
+  uint32_t* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(rayt::g_scratch_mutex);
+    base = rayt::g_scratch_ptr[ctx->device];
+  }
+  RT_REQUIRE(base, RT_ERR_STATE, "no scratch address for this device (context creation sets it up)");
+  rayt::SortPasses(ctx->stream, static_cast<int*>(data->ptr), n, reinterpret_cast<int*>(base),
+                   base + rayt::kNoAllocSortMax);
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
 }
 
 int rt_scan_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp) {

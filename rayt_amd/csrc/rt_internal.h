@@ -80,6 +80,11 @@ int OrderTiles(rt_ctx* ctx, const TileSchedule& sched, int width, int height);
 // Tiles of one plain launch at the context's launch shape, and the workgroup size in waves.
 void PlainTiles(const rt_ctx* ctx, int width, int height, uint32_t* tiles, int* waves_per_tile);
 
+// Looks up, once per device, the device address of the static scratch array
+// of the sort that allocates nothing (rt_sort_i32_noalloc). Called by
+// rt_ctx_create with `device` current.
+int NoAllocScratchInit(int device);
+
 // A tracer whose last frame was pipelined (rt_tracer_render_frame on a fully
 // resident pool) still owes that frame's host half: applying its feedback
 // lists to the cache's LRU and filling its statistics. The cache and the

@@ -88,6 +88,11 @@ int rt_ctx_create(int device, rt_ctx** out) {
     return rayt::Fail(RT_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
   }
   c->compute_units = props.multiProcessorCount;
+  const int scratch_rc = rayt::NoAllocScratchInit(device);
+  if (scratch_rc != RT_OK) {
+    delete c;
+    return scratch_rc;
+  }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
