@@ -188,8 +188,19 @@ typedef struct rt_terrain_params {
 int rt_tree_open(const char* path, rt_tree** out);
 /* WriteTestOctreeSphere + BuildOctree, in memory: scene sources/stored-octree-test.cpp:317-323. */
 int rt_tree_build_sphere(float cx, float cy, float cz, float radius, int level, int nodes_in_block, rt_tree** out);
-/* Seeded fBm terrain + spheres + boxes built through the same blocking builder. */
+/* Seeded fBm terrain + spheres + boxes, blocked as StoredOctreeBuilder blocks a tree
+ * (CPU data/stored-octree-builder-impl.h:446-459, 348-411), into a tree held in host RAM. The subtrees
+ * below the root block are generated and blocked by params->threads workers in parallel and numbered
+ * as the sequential builder numbers them, so the node tree is never held whole (depth 15: ~5 G nodes). */
 int rt_tree_build_terrain(const rt_terrain_params* params, int nodes_in_block, rt_tree** out);
+/* The same blocks, written to a .tree file as they are numbered (StoredOctreeRawWriter,
+ * CPU data/stored-octree-raw-writer.cpp:10-99) instead of held in RAM; *out reads that file back
+ * (rt_tree_open). */
+int rt_tree_build_terrain_file(const rt_terrain_params* params, int nodes_in_block, const char* path,
+                               rt_tree** out);
+/* The same blocks from the whole node tree generated level by level in host RAM first (depth <= 14);
+ * no reference counterpart: the byte-for-byte check of the parallel builder. */
+int rt_tree_build_terrain_levelwise(const rt_terrain_params* params, int nodes_in_block, rt_tree** out);
 /* ImportObjScene (scene sources/import-obj-scene.cpp:640-652) in memory: reads a Wavefront OBJ with
  * its MTL libraries and PNM textures, voxelises it to depth max_level with channels "normals" (char4)
  * and "colors" (uchar4) (BuilderDataSource, :529-638) and blocks it like BuildOctree; rt_tree_write

@@ -81,6 +81,8 @@ SIGNATURES = {
     "rt_tree_open": (I, [C.c_char_p, PP]),
     "rt_tree_build_sphere": (I, [F, F, F, F, I, I, PP]),
     "rt_tree_build_terrain": (I, [C.POINTER(TerrainParams), I, PP]),
+    "rt_tree_build_terrain_file": (I, [C.POINTER(TerrainParams), I, C.c_char_p, PP]),
+    "rt_tree_build_terrain_levelwise": (I, [C.POINTER(TerrainParams), I, PP]),
     "rt_tree_import_obj": (I, [C.c_char_p, I, I, I, PP]),
     "rt_tree_write": (I, [P, C.c_char_p]),
     "rt_tree_info_get": (I, [P, C.POINTER(TreeInfo)]),

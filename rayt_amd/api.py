@@ -147,12 +147,33 @@ class StoredOctree:
         return StoredOctree(h)
 
     @staticmethod
+    def _params(spec: TerrainSpec):
+        return L.TerrainParams(spec.depth, spec.seed, spec.octaves, spec.height_lo, spec.height_hi, spec.spheres,
+                               spec.sphere_rmin, spec.sphere_rmax, spec.boxes, spec.box_emin, spec.box_emax,
+                               spec.threads)
+
+    @staticmethod
     def terrain(spec: TerrainSpec, nodes_in_block: int = 8192) -> "StoredOctree":
-        p = L.TerrainParams(spec.depth, spec.seed, spec.octaves, spec.height_lo, spec.height_hi, spec.spheres,
-                            spec.sphere_rmin, spec.sphere_rmax, spec.boxes, spec.box_emin, spec.box_emax,
-                            spec.threads)
+        """The terrain scene blocked by the parallel builder into host RAM."""
+        p = StoredOctree._params(spec)
         h = C.c_void_p()
         call("rt_tree_build_terrain", C.byref(p), nodes_in_block, C.byref(h))
+        return StoredOctree(h)
+
+    @staticmethod
+    def terrain_file(spec: TerrainSpec, path: str, nodes_in_block: int = 8192) -> "StoredOctree":
+        """The terrain scene blocked straight into the .tree file `path`, opened as a file-backed source."""
+        p = StoredOctree._params(spec)
+        h = C.c_void_p()
+        call("rt_tree_build_terrain_file", C.byref(p), nodes_in_block, path.encode(), C.byref(h))
+        return StoredOctree(h)
+
+    @staticmethod
+    def terrain_levelwise(spec: TerrainSpec, nodes_in_block: int = 8192) -> "StoredOctree":
+        """The same blocks from the whole node tree generated in RAM first (depth <= 14)."""
+        p = StoredOctree._params(spec)
+        h = C.c_void_p()
+        call("rt_tree_build_terrain_levelwise", C.byref(p), nodes_in_block, C.byref(h))
         return StoredOctree(h)
 
     @staticmethod

@@ -28,4 +28,17 @@ struct BuildReport {
 std::unique_ptr<MemoryTree> BuildOctree(const NodeTree& tree, int nodes_in_block, BuildReport* report,
                                         std::string* err);
 
+// Builds the terrain scene of `p` block by block, in parallel: the root
+// block is filled from the top of the tree; the subtree below each of its
+// unexpanded nodes is generated and blocked by one of p.threads workers,
+// numbering its blocks from 0; the sequential builder's processing of the
+// root block is then replayed in its own order with those results, giving
+// each worker's blocks the indices the sequential build gives them. The
+// blocks go to `sink` as soon as they are numbered (workers run at most a
+// few subtrees ahead), so the whole node tree is never held in memory, and
+// they equal BuildOctree(GenerateTerrainTree(p)) byte for byte. *header gets
+// the file header (block count included).
+bool BuildTerrainBlocks(const TerrainParams& p, int nodes_in_block, BlockSink* sink, StoredOctreeHeader* header,
+                        BuildReport* report, std::string* err);
+
 }  // namespace rayt

@@ -40,6 +40,35 @@ int FinishBuild(const rayt::NodeTree& t, int nodes_in_block, rt_tree** out) {
   return RT_OK;
 }
 
+int CheckTerrain(const rt_terrain_params* p, int max_depth, rayt::TerrainParams* tp) {
+  RT_REQUIRE(p, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(p->depth >= 1 && p->depth <= max_depth, RT_ERR_RANGE,
+             "terrain depth must be in [1, " + std::to_string(max_depth) + "]");
+  RT_REQUIRE(p->octaves >= 1 && p->octaves <= 16, RT_ERR_RANGE, "octaves must be in [1, 16]");
+  RT_REQUIRE(p->spheres >= 0 && p->boxes >= 0 && p->spheres + p->boxes <= 127, RT_ERR_RANGE,
+             "at most 127 spheres + boxes");
+  tp->depth = p->depth;
+  tp->seed = p->seed;
+  tp->octaves = p->octaves;
+  tp->height_lo = p->height_lo;
+  tp->height_hi = p->height_hi;
+  tp->spheres = p->spheres;
+  tp->sphere_rmin = p->sphere_rmin;
+  tp->sphere_rmax = p->sphere_rmax;
+  tp->boxes = p->boxes;
+  tp->box_emin = p->box_emin;
+  tp->box_emax = p->box_emax;
+  tp->threads = p->threads;
+  return RT_OK;
+}
+
+int FinishReport(const rayt::BuildReport& rep, BuildClock::time_point t0) {
+  BuildLog("terrain blocks (parallel builder)", rep.blocks, t0);
+  BuildLog("terrain nodes", rep.nodes, t0);
+  RT_REQUIRE(rep.allocated_groups == rep.deallocated_groups, RT_ERR_STATE, "builder leaked sibling groups");
+  return RT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -66,29 +95,63 @@ int rt_tree_build_sphere(float cx, float cy, float cz, float radius, int level, 
 }
 
 int rt_tree_build_terrain(const rt_terrain_params* p, int nodes_in_block, rt_tree** out) {
-  RT_REQUIRE(p && out, RT_ERR_INVALID, "null argument");
-  RT_REQUIRE(p->depth >= 1 && p->depth <= 15, RT_ERR_RANGE, "terrain depth must be in [1, 15]");
-  RT_REQUIRE(p->octaves >= 1 && p->octaves <= 16, RT_ERR_RANGE, "octaves must be in [1, 16]");
-  RT_REQUIRE(p->spheres >= 0 && p->boxes >= 0 && p->spheres + p->boxes <= 127, RT_ERR_RANGE,
-             "at most 127 spheres + boxes");
-  RT_TRY_BEGIN
+  RT_REQUIRE(out, RT_ERR_INVALID, "null out");
   rayt::TerrainParams tp;
-  tp.depth = p->depth;
-  tp.seed = p->seed;
-  tp.octaves = p->octaves;
-  tp.height_lo = p->height_lo;
-  tp.height_hi = p->height_hi;
-  tp.spheres = p->spheres;
-  tp.sphere_rmin = p->sphere_rmin;
-  tp.sphere_rmax = p->sphere_rmax;
-  tp.boxes = p->boxes;
-  tp.box_emin = p->box_emin;
-  tp.box_emax = p->box_emax;
-  tp.threads = p->threads;
+  int rc = CheckTerrain(p, 15, &tp);
+  if (rc) return rc;
+  RT_TRY_BEGIN
+  const auto t0 = BuildClock::now();
+  std::unique_ptr<rayt::MemoryTree> mt(new rayt::MemoryTree());
+  rayt::StoredOctreeHeader h;
+  rayt::BuildReport rep;
+  std::string err;
+  if (!rayt::BuildTerrainBlocks(tp, nodes_in_block, mt.get(), &h, &rep, &err))
+    return rayt::Fail(RT_ERR_INVALID, "build failed: " + err);
+  if ((rc = FinishReport(rep, t0))) return rc;
+  mt->mutable_header() = h;
+  mt->total_nodes = rep.nodes;
+  rt_tree* t = new rt_tree();
+  t->mem = mt.get();
+  t->src = std::move(mt);
+  *out = t;
+  return RT_OK;
+  RT_TRY_END
+}
+
+int rt_tree_build_terrain_levelwise(const rt_terrain_params* p, int nodes_in_block, rt_tree** out) {
+  RT_REQUIRE(out, RT_ERR_INVALID, "null out");
+  rayt::TerrainParams tp;
+  int rc = CheckTerrain(p, 14, &tp);
+  if (rc) return rc;
+  RT_TRY_BEGIN
   const auto t0 = BuildClock::now();
   rayt::NodeTree t = rayt::GenerateTerrainTree(tp);
   BuildLog("terrain nodes", t.size(), t0);
   return FinishBuild(t, nodes_in_block, out);
+  RT_TRY_END
+}
+
+int rt_tree_build_terrain_file(const rt_terrain_params* p, int nodes_in_block, const char* path, rt_tree** out) {
+  RT_REQUIRE(out && path, RT_ERR_INVALID, "null argument");
+  rayt::TerrainParams tp;
+  int rc = CheckTerrain(p, 15, &tp);
+  if (rc) return rc;
+  RT_TRY_BEGIN
+  const auto t0 = BuildClock::now();
+  rayt::StoredOctreeHeader h;
+  h.nodes_in_block = uint32_t(nodes_in_block);
+  h.channels.push_back({rayt::kNodeLinkSize, rayt::kNodeLinkChannelName});
+  for (auto& c : rayt::TerrainSource::Channels()) h.channels.push_back(c);
+  rayt::TreeFileWriter w(path, h);
+  RT_REQUIRE(w.ok(), RT_ERR_IO, std::string("cannot create ") + path);
+  rayt::BuildReport rep;
+  std::string err;
+  rayt::StoredOctreeHeader built;
+  if (!rayt::BuildTerrainBlocks(tp, nodes_in_block, &w, &built, &rep, &err))
+    return rayt::Fail(RT_ERR_IO, "build failed: " + err);
+  if (!w.Finish(built.blocks_count, &err)) return rayt::Fail(RT_ERR_IO, err);
+  if ((rc = FinishReport(rep, t0))) return rc;
+  return rt_tree_open(path, out);
   RT_TRY_END
 }
 

@@ -144,6 +144,45 @@ void MemoryTree::PutBlock(const StoredOctreeBlockHeader& h, const uint8_t* paylo
   records_[h.block_index] = std::move(rec);
 }
 
+bool MemoryTree::Put(uint32_t index, std::unique_ptr<uint8_t[]> record) {
+  if (records_.size() <= index) records_.resize(size_t(index) + 1);
+  records_[index] = std::move(record);
+  return true;
+}
+
+TreeFileWriter::TreeFileWriter(const std::string& path, const StoredOctreeHeader& header) : header_(header) {
+  f_ = fopen(path.c_str(), "w+b");
+  record_size_ = kBlockHeaderSize + uint64_t(header.SumBytesInNode()) * header.nodes_in_block;
+  header_.block_stride = static_cast<uint32_t>(record_size_);
+  header_size_ = EncodeFileHeader(header_).size();
+}
+
+TreeFileWriter::~TreeFileWriter() {
+  if (f_) fclose(f_);
+}
+
+bool TreeFileWriter::Put(uint32_t index, std::unique_ptr<uint8_t[]> record) {
+  if (!f_ || failed_) return false;
+  const uint64_t pos = header_size_ + uint64_t(index) * record_size_;
+  if (fseeko(f_, off_t(pos), SEEK_SET) != 0 || fwrite(record.get(), 1, record_size_, f_) != record_size_)
+    failed_ = true;
+  return !failed_;
+}
+
+bool TreeFileWriter::Finish(uint32_t blocks_count, std::string* err) {
+  if (!f_ || failed_) {
+    if (err) *err = "writing the tree file failed";
+    return false;
+  }
+  header_.blocks_count = blocks_count;
+  const std::vector<uint8_t> hb = EncodeFileHeader(header_);
+  bool ok = fseeko(f_, 0, SEEK_SET) == 0 && fwrite(hb.data(), 1, hb.size(), f_) == hb.size();
+  ok = (fclose(f_) == 0) && ok;
+  f_ = nullptr;
+  if (!ok && err) *err = "writing the tree file header failed";
+  return ok;
+}
+
 bool MemoryTree::WriteFile(const std::string& path, std::string* err) const {
   FILE* f = fopen(path.c_str(), "wb");
   if (!f) {

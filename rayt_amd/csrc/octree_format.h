@@ -171,10 +171,36 @@ class StoredOctreeLoader : public BlockSource {
   uint64_t header_size_ = 0;
 };
 
+// Where a builder puts its blocks: one record (76-byte header + payload) per
+// block index, in any order. Put returns false on failure (e.g. I/O).
+class BlockSink {
+ public:
+  virtual ~BlockSink() = default;
+  virtual bool Put(uint32_t index, std::unique_ptr<uint8_t[]> record) = 0;
+};
+
+// Writes block records straight into a .tree file at their final offsets
+// (header + index * stride), so a build need not hold the tree in memory.
+// Finish writes the file header once the block count is known.
+class TreeFileWriter : public BlockSink {
+ public:
+  TreeFileWriter(const std::string& path, const StoredOctreeHeader& header);
+  ~TreeFileWriter() override;
+  bool ok() const { return f_ != nullptr; }
+  bool Put(uint32_t index, std::unique_ptr<uint8_t[]> record) override;
+  bool Finish(uint32_t blocks_count, std::string* err);
+
+ private:
+  FILE* f_ = nullptr;
+  StoredOctreeHeader header_;
+  uint64_t header_size_ = 0, record_size_ = 0;
+  bool failed_ = false;
+};
+
 // A complete tree in host RAM, one record per block in file layout. This is
 // the host-side block store for out-of-core streaming, and the builder's
 // output. WriteFile produces a .tree file readable by StoredOctreeLoader.
-class MemoryTree : public BlockSource {
+class MemoryTree : public BlockSource, public BlockSink {
  public:
   MemoryTree() = default;
   const StoredOctreeHeader& header() const override { return header_; }
@@ -186,6 +212,7 @@ class MemoryTree : public BlockSource {
   // Record layout: 76-byte header + payload (BytesInBlockContent()).
   uint64_t record_size() const { return kBlockHeaderSize + BytesInBlockContent(); }
   void PutBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload);
+  bool Put(uint32_t index, std::unique_ptr<uint8_t[]> record) override;
   bool WriteFile(const std::string& path, std::string* err) const;
   uint64_t total_nodes = 0;  // nodes in the source tree (informational)
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
 #include <thread>
 
 namespace rayt {
@@ -118,23 +119,56 @@ static inline double LatticeValue(uint64_t seed, int64_t ix, int64_t iz, int oct
 }
 static inline double Smooth(double t) { return t * t * (3.0 - 2.0 * t); }
 
-// Seeded value-noise fBm (lacunarity 2, gain 0.5).
-static double Fbm(uint64_t seed, double x, double z, int octaves, double base_freq) {
-  double sum = 0, amp = 1, freq = base_freq;
-  for (int o = 0; o < octaves; ++o) {
-    double fx = x * freq, fz = z * freq;
-    double flx = std::floor(fx), flz = std::floor(fz);
-    int64_t ix = int64_t(flx), iz = int64_t(flz);
-    double tx = Smooth(fx - flx), tz = Smooth(fz - flz);
-    double v00 = LatticeValue(seed, ix, iz, o), v10 = LatticeValue(seed, ix + 1, iz, o);
-    double v01 = LatticeValue(seed, ix, iz + 1, o), v11 = LatticeValue(seed, ix + 1, iz + 1, o);
-    double a = v00 + (v10 - v00) * tx, b = v01 + (v11 - v01) * tx;
-    sum += amp * (a + (b - a) * tz);
-    amp *= 0.5;
-    freq *= 2.0;
+// Seeded value-noise fBm (lacunarity 2, gain 0.5) over x, z in [0, 1]. The
+// lattice values of each octave whose lattice is small (at most 2^22 points)
+// are tabulated once; larger lattices are hashed per lookup. Both give the
+// same values.
+class Fbm {
+ public:
+  Fbm(uint64_t seed, int octaves, double base_freq) : seed_(seed), octaves_(octaves), base_freq_(base_freq) {
+    double freq = base_freq;
+    for (int o = 0; o < octaves; ++o, freq *= 2.0) {
+      const int64_t w = int64_t(std::floor(freq)) + 2;  // lattice points 0 .. floor(freq) + 1 per axis
+      width_.push_back(w);
+      tab_.emplace_back();
+      if (w * w > (int64_t(1) << 22)) continue;
+      tab_.back().resize(size_t(w * w));
+      for (int64_t iz = 0; iz < w; ++iz)
+        for (int64_t ix = 0; ix < w; ++ix) tab_.back()[size_t(iz * w + ix)] = LatticeValue(seed, ix, iz, o);
+    }
   }
-  return sum;
-}
+  double operator()(double x, double z) const {
+    double sum = 0, amp = 1, freq = base_freq_;
+    for (int o = 0; o < octaves_; ++o) {
+      double fx = x * freq, fz = z * freq;
+      double flx = std::floor(fx), flz = std::floor(fz);
+      int64_t ix = int64_t(flx), iz = int64_t(flz);
+      double tx = Smooth(fx - flx), tz = Smooth(fz - flz);
+      double v00, v10, v01, v11;
+      const std::vector<double>& t = tab_[o];
+      const int64_t w = width_[o];
+      if (!t.empty() && ix >= 0 && iz >= 0 && ix + 1 < w && iz + 1 < w) {
+        const double* r0 = &t[size_t(iz * w + ix)];
+        v00 = r0[0], v10 = r0[1], v01 = r0[w], v11 = r0[w + 1];
+      } else {
+        v00 = LatticeValue(seed_, ix, iz, o), v10 = LatticeValue(seed_, ix + 1, iz, o);
+        v01 = LatticeValue(seed_, ix, iz + 1, o), v11 = LatticeValue(seed_, ix + 1, iz + 1, o);
+      }
+      double a = v00 + (v10 - v00) * tx, b = v01 + (v11 - v01) * tx;
+      sum += amp * (a + (b - a) * tz);
+      amp *= 0.5;
+      freq *= 2.0;
+    }
+    return sum;
+  }
+
+ private:
+  uint64_t seed_;
+  int octaves_;
+  double base_freq_;
+  std::vector<int64_t> width_;
+  std::vector<std::vector<double>> tab_;
+};
 
 struct Sphere {
   double c[3], r;
@@ -159,14 +193,10 @@ static void ParallelFor(int threads, size_t n, F&& f) {
   for (auto& x : th) x.join();
 }
 
-struct LevelNode {
-  uint32_t i, j, k;    // integer cell coordinates at this level
-  uint64_t prims[2];   // primitives whose surface the cell straddles: bit 0 terrain, 1..8 spheres, then boxes
-};
+}  // namespace
 
-class TerrainBuilder {
- public:
-  explicit TerrainBuilder(const TerrainParams& p) : p_(p) {
+struct TerrainSource::Impl {
+  explicit Impl(const TerrainParams& p) : p_(p) {
     threads_ = p.threads > 0 ? p.threads : std::max(1u, std::thread::hardware_concurrency());
     N_ = 1u << p.depth;
     uint64_t s = p.seed * 0x2545F4914F6CDD1Dull + 0x1234567ull;
@@ -185,12 +215,10 @@ class TerrainBuilder {
       }
       boxes_.push_back(b);
     }
+    channels_ = TerrainSource::Channels();
     BuildHeightmap();
   }
 
-  NodeTree Build();
-
- private:
   TerrainParams p_;
   int threads_;
   uint32_t N_;
@@ -198,18 +226,19 @@ class TerrainBuilder {
   std::vector<std::vector<float>> hmin_, hmax_;  // per level L: 2^L x 2^L cells
   std::vector<Sphere> spheres_;
   std::vector<Box> boxes_;
+  std::vector<StoredOctreeChannel> channels_;
 
   float Hat(uint32_t i, uint32_t k) const { return H_[size_t(k) * (N_ + 1) + i]; }
+  int nprims() const { return 1 + int(spheres_.size()) + int(boxes_.size()); }
 
   void BuildHeightmap() {
     const uint32_t M = N_ + 1;
     H_.assign(size_t(M) * M, 0.f);
     std::vector<double> raw(size_t(M) * M);
-    uint64_t nseed = p_.seed * 0x9E3779B97F4A7C15ull + 77;
+    const Fbm fbm(p_.seed * 0x9E3779B97F4A7C15ull + 77, p_.octaves, p_.base_frequency);
     ParallelFor(threads_, M, [&](size_t b, size_t e, int) {
       for (size_t k = b; k < e; ++k)
-        for (uint32_t i = 0; i < M; ++i)
-          raw[k * M + i] = Fbm(nseed, double(i) / N_, double(k) / N_, p_.octaves, p_.base_frequency);
+        for (uint32_t i = 0; i < M; ++i) raw[k * M + i] = fbm(double(i) / N_, double(k) / N_);
     });
     double lo = raw[0], hi = raw[0];
     for (double v : raw) lo = std::min(lo, v), hi = std::max(hi, v);
@@ -221,12 +250,15 @@ class TerrainBuilder {
     hmax_.resize(D + 1);
     hmin_[D].resize(size_t(N_) * N_);
     hmax_[D].resize(size_t(N_) * N_);
-    for (uint32_t k = 0; k < N_; ++k)
-      for (uint32_t i = 0; i < N_; ++i) {
-        float a = Hat(i, k), b = Hat(i + 1, k), c = Hat(i, k + 1), d = Hat(i + 1, k + 1);
-        hmin_[D][size_t(k) * N_ + i] = std::min(std::min(a, b), std::min(c, d));
-        hmax_[D][size_t(k) * N_ + i] = std::max(std::max(a, b), std::max(c, d));
-      }
+    ParallelFor(threads_, N_, [&](size_t b, size_t e, int) {
+      for (size_t k = b; k < e; ++k)
+        for (uint32_t i = 0; i < N_; ++i) {
+          float a = Hat(i, uint32_t(k)), bb = Hat(i + 1, uint32_t(k)), c = Hat(i, uint32_t(k) + 1),
+                d = Hat(i + 1, uint32_t(k) + 1);
+          hmin_[D][k * N_ + i] = std::min(std::min(a, bb), std::min(c, d));
+          hmax_[D][k * N_ + i] = std::max(std::max(a, bb), std::max(c, d));
+        }
+    });
     for (int L = D - 1; L >= 0; --L) {
       uint32_t n = 1u << L;
       hmin_[L].resize(size_t(n) * n);
@@ -321,152 +353,200 @@ class TerrainBuilder {
     }
     for (int c = 0; c < 3; ++c) rgb[c] = float(stops[3][c + 1]);
   }
+
+  int Children(const TerrainCell& n, TerrainCell out[8], uint8_t* mask) const {
+    *mask = 0;
+    if (n.level >= p_.depth) return 0;
+    int cnt = 0;
+    // the height field's test (Straddles, prim 0) inlined: the children share
+    // four footprints of the level's min/max pyramid
+    const int L = n.level + 1;
+    const double sc = std::ldexp(1.0, -L);
+    const size_t w = size_t(1) << L;
+    const float* mn = hmin_[L].data();
+    const float* mx = hmax_[L].data();
+    const bool terrain = n.prims[0] & 1u;
+    for (int ch = 0; ch < 8; ++ch) {
+      TerrainCell c;
+      c.i = 2 * n.i + (ch & 1);
+      c.j = 2 * n.j + ((ch >> 1) & 1);
+      c.k = 2 * n.k + ((ch >> 2) & 1);
+      c.level = L;
+      if (terrain) {
+        const size_t f = size_t(c.k) * w + c.i;
+        if (double(mn[f]) <= (c.j + 1) * sc && double(mx[f]) >= c.j * sc) c.prims[0] |= 1u;
+      }
+      for (int wd = 0; wd < 2; ++wd) {
+        uint64_t m = n.prims[wd] & (wd == 0 ? ~uint64_t(1) : ~uint64_t(0));
+        while (m) {
+          int bit = __builtin_ctzll(m);
+          m &= m - 1;
+          if (Straddles(wd * 64 + bit, c.i, c.j, c.k, c.level)) c.prims[wd] |= 1ull << bit;
+        }
+      }
+      if (c.prims[0] | c.prims[1]) {
+        *mask |= uint8_t(1u << ch);
+        out[cnt++] = c;
+      }
+    }
+    return cnt;
+  }
+
+  // Leaves take the nearest surface among the primitives they straddle.
+  void LeafValues(const TerrainCell& c, float nc[6]) const {
+    const double s = std::ldexp(1.0, -c.level);
+    double p[3] = {(c.i + 0.5) * s, (c.j + 0.5) * s, (c.k + 0.5) * s};
+    double best = 1e300, bn[3] = {0, 0, 1};
+    uint32_t fi = c.i << (p_.depth - c.level), fk = c.k << (p_.depth - c.level);
+    for (int w = 0; w < 2; ++w) {
+      uint64_t mm = c.prims[w];
+      while (mm) {
+        int q = w * 64 + __builtin_ctzll(mm);
+        mm &= mm - 1;
+        double n[3];
+        double d = SurfaceDistance(q, p, fi, fk, n);
+        if (d < best) best = d, bn[0] = n[0], bn[1] = n[1], bn[2] = n[2];
+      }
+    }
+    double len = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
+    for (int a = 0; a < 3; ++a) nc[a] = len > 0 ? float(bn[a] / len) : (a == 2 ? 1.f : 0.f);
+    Gradient(p[1], nc + 3);
+  }
+
+  NodeTree Subtree(const TerrainCell& top, int threads, float root_values[6]) const {
+    NodeTree t;
+    t.max_depth = p_.depth;
+    t.channels = channels_;
+    t.data_size = 8;
+    const int L0 = top.level, D = p_.depth;
+    // Level-synchronous expansion. levels[L - L0] holds the cells of level L in BFS order.
+    std::vector<std::vector<TerrainCell>> levels(D - L0 + 1);
+    levels[0].push_back(top);
+    std::vector<std::vector<uint8_t>> masks(D - L0 + 1);
+    for (int L = L0; L < D; ++L) {
+      const auto& cur = levels[L - L0];
+      masks[L - L0].assign(cur.size(), 0);
+      std::vector<std::vector<TerrainCell>> parts(threads);
+      ParallelFor(threads, cur.size(), [&](size_t b, size_t e, int tid) {
+        auto& out = parts[tid];
+        TerrainCell kids[8];
+        for (size_t x = b; x < e; ++x) {
+          const int n = Children(cur[x], kids, &masks[L - L0][x]);
+          out.insert(out.end(), kids, kids + n);
+        }
+      });
+      size_t total = 0;
+      for (auto& pp : parts) total += pp.size();
+      levels[L + 1 - L0].reserve(total);
+      for (auto& pp : parts) levels[L + 1 - L0].insert(levels[L + 1 - L0].end(), pp.begin(), pp.end());
+      if (threads > 1 && std::getenv("RAYT_BUILD_LOG")) {  // progress of long builds
+        std::fprintf(stderr, "[rayt build] level %d: %zu cells\n", L + 1, total);
+        std::fflush(stderr);
+      }
+    }
+    masks[D - L0].assign(levels[D - L0].size(), 0);
+
+    // Flatten: node index = level offset + position; children contiguous.
+    std::vector<uint64_t> level_off(D - L0 + 2, 0);
+    for (int L = L0; L <= D; ++L) level_off[L - L0 + 1] = level_off[L - L0] + levels[L - L0].size();
+    const uint64_t total = level_off[D - L0 + 1];
+    if (total > 0xffffffffull) throw std::runtime_error("subtree has more than 2^32 nodes");
+    t.mask.resize(total);
+    t.first_child.resize(total);
+    t.data.resize(total * 8);
+    for (int L = L0; L <= D; ++L) {
+      uint64_t next = level_off[L - L0 + 1];
+      for (size_t x = 0; x < levels[L - L0].size(); ++x) {
+        uint8_t m = masks[L - L0][x];
+        t.mask[level_off[L - L0] + x] = m;
+        t.first_child[level_off[L - L0] + x] = m ? uint32_t(next) : 0;
+        next += __builtin_popcount(m);
+      }
+    }
+
+    // Node data bottom-up: leaves from the nearest primitive surface; inner
+    // nodes average their children (normal sum re-normalised, colour mean).
+    std::vector<float> child_nc, cur_nc;  // 6 floats per node (normal xyz, colour rgb)
+    for (int L = D; L >= L0; --L) {
+      const auto& lv = levels[L - L0];
+      cur_nc.assign(lv.size() * 6, 0.f);
+      ParallelFor(threads, lv.size(), [&](size_t b, size_t e, int) {
+        for (size_t x = b; x < e; ++x) {
+          uint64_t node = level_off[L - L0] + x;
+          float* nc = &cur_nc[x * 6];
+          uint8_t m = t.mask[node];
+          if (!m)
+            LeafValues(lv[x], nc);
+          else
+            InnerValues(&child_nc[(t.first_child[node] - level_off[L - L0 + 1]) * 6], __builtin_popcount(m), nc);
+          PackValues(nc, &t.data[node * 8]);
+        }
+      });
+      std::swap(child_nc, cur_nc);
+    }
+    for (int a = 0; a < 6; ++a) root_values[a] = child_nc[a];
+    return t;
+  }
+
+  static void InnerValues(const float* children, int cnt, float nc[6]) {
+    float n[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    for (int q = 0; q < cnt; ++q) {
+      const float* cc = &children[q * 6];
+      for (int a = 0; a < 3; ++a) n[a] += cc[a], c[a] += cc[3 + a];
+    }
+    float inv = 1.0f / float(cnt);
+    for (int a = 0; a < 3; ++a) c[a] *= inv;
+    float ls = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (ls < 1e-5f) {
+      n[0] = 0; n[1] = 0; n[2] = 1;
+    } else {
+      float il = 1.0f / std::sqrt(ls);
+      for (int a = 0; a < 3; ++a) n[a] *= il;
+    }
+    for (int a = 0; a < 3; ++a) nc[a] = n[a], nc[3 + a] = c[a];
+  }
+
+  // Quantised channel bytes: normal char4 = (char)(n*127), w 0; colour uchar4 =
+  // (uchar)clamp(c*255), alpha 255 (import-obj-scene.cpp:516-526).
+  static void PackValues(const float v[6], uint8_t* out) {
+    for (int a = 0; a < 3; ++a) out[a] = static_cast<uint8_t>(static_cast<char>(v[a] * 127));
+    out[3] = 0;
+    for (int a = 0; a < 3; ++a)
+      out[4 + a] = static_cast<uint8_t>(std::max(0.0f, std::min(255.0f, v[3 + a] * 255)));
+    out[7] = 255;
+  }
+
+  TerrainCell Root() const {
+    TerrainCell r;
+    for (int q = 0; q < nprims(); ++q)
+      if (Straddles(q, 0, 0, 0, 0)) r.prims[q >> 6] |= 1ull << (q & 63);
+    return r;
+  }
 };
 
-// Quantised channel bytes: normal char4 = (char)(n*127), w 0; colour uchar4 =
-// (uchar)clamp(c*255), alpha 255 (import-obj-scene.cpp:516-526).
-static inline void PackNodeData(const float n[3], const float c[3], uint8_t* out) {
-  for (int a = 0; a < 3; ++a) out[a] = static_cast<uint8_t>(static_cast<char>(n[a] * 127));
-  out[3] = 0;
-  for (int a = 0; a < 3; ++a)
-    out[4 + a] = static_cast<uint8_t>(std::max(0.0f, std::min(255.0f, c[a] * 255)));
-  out[7] = 255;
+TerrainSource::TerrainSource(const TerrainParams& p) : impl_(new Impl(p)) {}
+TerrainSource::~TerrainSource() { delete impl_; }
+int TerrainSource::depth() const { return impl_->p_.depth; }
+int TerrainSource::threads() const { return impl_->threads_; }
+const std::vector<StoredOctreeChannel>& TerrainSource::channels() const { return impl_->channels_; }
+std::vector<StoredOctreeChannel> TerrainSource::Channels() { return {{4, "normals"}, {4, "colors"}}; }
+TerrainCell TerrainSource::Root() const { return impl_->Root(); }
+int TerrainSource::Children(const TerrainCell& c, TerrainCell out[8], uint8_t* mask) const {
+  return impl_->Children(c, out, mask);
 }
-
-NodeTree TerrainBuilder::Build() {
-  NodeTree t;
-  t.max_depth = p_.depth;
-  t.channels = {{4, "normals"}, {4, "colors"}};
-  t.data_size = 8;
-  const int nprims = 1 + int(spheres_.size()) + int(boxes_.size());
-
-  // Level-synchronous expansion. levels[L] holds the cells of level L in BFS order.
-  std::vector<std::vector<LevelNode>> levels(p_.depth + 1);
-  {
-    LevelNode root{0, 0, 0, {0, 0}};
-    for (int q = 0; q < nprims; ++q)
-      if (Straddles(q, 0, 0, 0, 0)) root.prims[q >> 6] |= 1ull << (q & 63);
-    levels[0].push_back(root);
-  }
-  std::vector<std::vector<uint8_t>> masks(p_.depth + 1);
-  for (int L = 0; L < p_.depth; ++L) {
-    const auto& cur = levels[L];
-    masks[L].assign(cur.size(), 0);
-    std::vector<std::vector<LevelNode>> parts(threads_);
-    ParallelFor(threads_, cur.size(), [&](size_t b, size_t e, int tid) {
-      auto& out = parts[tid];
-      for (size_t x = b; x < e; ++x) {
-        const LevelNode& n = cur[x];
-        uint8_t mask = 0;
-        for (int ch = 0; ch < 8; ++ch) {
-          LevelNode c{2 * n.i + (ch & 1), 2 * n.j + ((ch >> 1) & 1), 2 * n.k + ((ch >> 2) & 1), {0, 0}};
-          for (int w = 0; w < 2; ++w) {
-            uint64_t m = n.prims[w];
-            while (m) {
-              int bit = __builtin_ctzll(m);
-              m &= m - 1;
-              int q = w * 64 + bit;
-              if (Straddles(q, c.i, c.j, c.k, L + 1)) c.prims[w] |= 1ull << bit;
-            }
-          }
-          if (c.prims[0] | c.prims[1]) {
-            mask |= uint8_t(1u << ch);
-            out.push_back(c);
-          }
-        }
-        masks[L][x] = mask;
-      }
-    });
-    size_t total = 0;
-    for (auto& pp : parts) total += pp.size();
-    levels[L + 1].reserve(total);
-    for (auto& pp : parts) levels[L + 1].insert(levels[L + 1].end(), pp.begin(), pp.end());
-    if (std::getenv("RAYT_BUILD_LOG")) {  // progress of long builds
-      std::fprintf(stderr, "[rayt build] level %d: %zu cells\n", L + 1, total);
-      std::fflush(stderr);
-    }
-  }
-  masks[p_.depth].assign(levels[p_.depth].size(), 0);
-
-  // Flatten: node index = level offset + position; children contiguous.
-  std::vector<uint64_t> level_off(p_.depth + 2, 0);
-  for (int L = 0; L <= p_.depth; ++L) level_off[L + 1] = level_off[L] + levels[L].size();
-  const uint64_t total = level_off[p_.depth + 1];
-  t.mask.resize(total);
-  t.first_child.resize(total);
-  t.data.resize(total * 8);
-  for (int L = 0; L <= p_.depth; ++L) {
-    uint64_t next = level_off[L + 1];
-    for (size_t x = 0; x < levels[L].size(); ++x) {
-      uint8_t m = masks[L][x];
-      t.mask[level_off[L] + x] = m;
-      t.first_child[level_off[L] + x] = m ? uint32_t(next) : 0;
-      next += __builtin_popcount(m);
-    }
-  }
-
-  // Node data bottom-up: leaves from the nearest primitive surface; inner
-  // nodes average their children (normal sum re-normalised, colour mean).
-  std::vector<float> child_nc, cur_nc;  // 6 floats per node (normal xyz, colour rgb)
-  for (int L = p_.depth; L >= 0; --L) {
-    const auto& lv = levels[L];
-    cur_nc.assign(lv.size() * 6, 0.f);
-    const double s = std::ldexp(1.0, -L);
-    ParallelFor(threads_, lv.size(), [&](size_t b, size_t e, int) {
-      for (size_t x = b; x < e; ++x) {
-        uint64_t node = level_off[L] + x;
-        float* nc = &cur_nc[x * 6];
-        uint8_t m = t.mask[node];
-        if (!m) {
-          double p[3] = {(lv[x].i + 0.5) * s, (lv[x].j + 0.5) * s, (lv[x].k + 0.5) * s};
-          // leaves: nearest primitive among those straddled
-          double best = 1e300, bn[3] = {0, 0, 1};
-          uint32_t fi = lv[x].i << (p_.depth - L), fk = lv[x].k << (p_.depth - L);
-          for (int w = 0; w < 2; ++w) {
-            uint64_t mm = lv[x].prims[w];
-            while (mm) {
-              int q = w * 64 + __builtin_ctzll(mm);
-              mm &= mm - 1;
-              double n[3];
-              double d = SurfaceDistance(q, p, fi, fk, n);
-              if (d < best) best = d, bn[0] = n[0], bn[1] = n[1], bn[2] = n[2];
-            }
-          }
-          double len = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
-          for (int a = 0; a < 3; ++a) nc[a] = len > 0 ? float(bn[a] / len) : (a == 2 ? 1.f : 0.f);
-          Gradient(p[1], nc + 3);
-        } else {
-          uint64_t c0 = t.first_child[node] - level_off[L + 1];
-          int cnt = __builtin_popcount(m);
-          float n[3] = {0, 0, 0}, c[3] = {0, 0, 0};
-          for (int q = 0; q < cnt; ++q) {
-            const float* cc = &child_nc[(c0 + q) * 6];
-            for (int a = 0; a < 3; ++a) n[a] += cc[a], c[a] += cc[3 + a];
-          }
-          float inv = 1.0f / float(cnt);
-          for (int a = 0; a < 3; ++a) c[a] *= inv;
-          float ls = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-          if (ls < 1e-5f) {
-            n[0] = 0; n[1] = 0; n[2] = 1;
-          } else {
-            float il = 1.0f / std::sqrt(ls);
-            for (int a = 0; a < 3; ++a) n[a] *= il;
-          }
-          for (int a = 0; a < 3; ++a) nc[a] = n[a], nc[3 + a] = c[a];
-        }
-        PackNodeData(nc, nc + 3, &t.data[node * 8]);
-      }
-    });
-    std::swap(child_nc, cur_nc);
-  }
-  return t;
+NodeTree TerrainSource::Subtree(const TerrainCell& c, int threads, float root_values[6]) const {
+  return impl_->Subtree(c, std::max(1, threads), root_values);
 }
-
-}  // namespace
+void TerrainSource::LeafValues(const TerrainCell& c, float out[6]) const { impl_->LeafValues(c, out); }
+void TerrainSource::InnerValues(const float* children, int count, float out[6]) {
+  Impl::InnerValues(children, count, out);
+}
+void TerrainSource::PackValues(const float v[6], uint8_t out[8]) { Impl::PackValues(v, out); }
 
 NodeTree GenerateTerrainTree(const TerrainParams& p) {
-  TerrainBuilder b(p);
-  return b.Build();
+  TerrainSource src(p);
+  float root[6];
+  return src.Subtree(src.Root(), src.threads(), root);
 }
 
 }  // namespace rayt

@@ -48,4 +48,46 @@ struct TerrainParams {
 
 NodeTree GenerateTerrainTree(const TerrainParams& p);
 
+// A cell of the terrain scene: integer coordinates at `level` and the
+// primitives whose surface it straddles (bit 0 the height field, then the
+// spheres, then the boxes).
+struct TerrainCell {
+  uint32_t i = 0, j = 0, k = 0;
+  int level = 0;
+  uint64_t prims[2] = {0, 0};
+};
+
+// The terrain scene as a node source: the children of any cell, and any
+// cell's subtree as a NodeTree. GenerateTerrainTree is Subtree(Root()).
+// Subtrees of different cells can be generated concurrently.
+class TerrainSource {
+ public:
+  explicit TerrainSource(const TerrainParams& p);
+  ~TerrainSource();
+  TerrainSource(const TerrainSource&) = delete;
+  TerrainSource& operator=(const TerrainSource&) = delete;
+  int depth() const;
+  int threads() const;
+  const std::vector<StoredOctreeChannel>& channels() const;
+  // The data channels of every terrain scene: "normals" (char4), "colors" (uchar4).
+  static std::vector<StoredOctreeChannel> Channels();
+  TerrainCell Root() const;
+  // Children in child-index order (cell (2i+(c&1), 2j+((c>>1)&1), 2k+(c>>2))
+  // for child c); returns their count and sets *mask. A cell at the full
+  // depth has none.
+  int Children(const TerrainCell& c, TerrainCell out[8], uint8_t* mask) const;
+  // The subtree of `c` (node 0 = c) with the node data bytes of the full tree;
+  // *root_values gets node 0's unquantised normal and colour (6 floats).
+  NodeTree Subtree(const TerrainCell& c, int threads, float root_values[6]) const;
+  // Unquantised normal and colour of a leaf cell, and of an inner node from
+  // its children's (in child order); the byte packing of the node data.
+  void LeafValues(const TerrainCell& c, float out[6]) const;
+  static void InnerValues(const float* children, int count, float out[6]);
+  static void PackValues(const float v[6], uint8_t out[8]);
+
+ private:
+  struct Impl;
+  Impl* impl_;
+};
+
 }  // namespace rayt
