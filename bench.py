@@ -9,13 +9,18 @@ the cache transaction that marks the hit blocks, as the reference does.
 
 Prints one JSON line (see the field list at the bottom). The CPU baseline is
 the oracle (a strict-fp32 C++ restatement of the same traversal) on the same
-frame, on up to 16 host threads.
+frame, on every host core. Beside the headline come the same frame under a
+moving camera, the 64-view batched mix, the served request stream (depths
+12-15 sharing a 16 GiB block pool, depth 15 served from its .tree file), the
+depth-14 out-of-core fly-through in an 8 GiB pool, an imported OBJ scene and
+the reference's CPU-sized case.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import shutil
 import sys
 import time
 
@@ -30,6 +35,16 @@ def log(msg: str):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "render_plain_depth11_1080p_hbm_traffic.md")
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def measured_traffic():
@@ -255,6 +270,32 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
             "burst": closed}
 
 
+def measure_plain_moving(ctx, cache, tr, W: int, H: int, flags: int, lod: float, frames: int = 64):
+    """The plain frame over a moving camera: the seeded 64-pose spline fly-through over the same resident scene
+    (a new view every frame, so the tile order learned from earlier frames is always one frame old). Rays per
+    frame come from the counting kernel, outside the timed region."""
+    from rayt_amd import api
+    from rayt_amd.workloads import camera, spline_poses
+    vpis = [camera(y, p, pos, W, H) for y, p, pos in spline_poses(frames, seed=5)]
+    rays = 0
+    for v in vpis:
+        _, _, st = api.render_plain(ctx, cache, W, H, v, lod=lod, flags=flags, count=True)
+        rays += st[0] + st[1]
+    for v in vpis[:4]:
+        tr.RenderFrame(v)
+    ms0, n0 = tr.trace_totals()
+    t0 = time.perf_counter()
+    for v in vpis:
+        tr.RenderFrame(v)
+    ms1, n1 = tr.trace_totals()
+    wall = time.perf_counter() - t0
+    res = {"workload": f"{frames}-pose seeded spline fly-through, same scene and resolution", "frames": frames,
+           "rays": rays, "ms_per_frame": round(wall / frames * 1e3, 4), "mrays_s": round(rays / wall / 1e6, 1),
+           "kernel_ms_per_frame": round((ms1 - ms0) / (n1 - n0), 4)}
+    log(f"plain frames, moving camera: {res}")
+    return res
+
+
 def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int, H: int, lod: float = 1.0,
                         pool_gb: float = 0.0):
     """Fly-through over a block pool that holds `frac` of the tree's blocks (or `pool_gb` GiB of blocks when
@@ -432,10 +473,14 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-burst", type=int, default=256, help="requests queued at once for the saturation run")
-    ap.add_argument("--serve-depths", default="9,10,11,12",
-                    help="depths of the served scenes (seeds 2, 3, ...; the depth-11 plain scene is reused)")
+    ap.add_argument("--serve-depths", default="12,13,14,15",
+                    help="depths of the served scenes (fBm terrain, 6 octaves, seed = depth)")
+    ap.add_argument("--serve-file-depth", type=int, default=15,
+                    help="served scenes at least this deep are built straight into a .tree file under --scene-dir "
+                         "and served from it (0: every scene in host RAM)")
+    ap.add_argument("--scene-dir", default="/tmp", help="where file-backed scenes are written (deleted afterwards)")
     ap.add_argument("--serve-pool-gb", type=float, default=16.0,
                     help="block-pool budget shared by the served scenes; scenes beyond it are served out of core")
     ap.add_argument("--serve-shape", default="0,12,0", metavar="TILE,LEVELS,WGS",
@@ -445,9 +490,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run a device-to-device copy of MB MiB (3 times) as the byte-counter reference")
     ap.add_argument("--no-outofcore", action="store_true")
-    ap.add_argument("--ooc-depth", type=int, default=12)
+    ap.add_argument("--ooc-depth", type=int, default=14)
     ap.add_argument("--ooc-frac", type=float, default=0.25, help="pool size as a fraction of the tree's blocks")
-    ap.add_argument("--ooc-pool-gb", type=float, default=0.0, help="pool size in GiB (instead of --ooc-frac)")
+    ap.add_argument("--ooc-pool-gb", type=float, default=8.0, help="pool size in GiB (0: --ooc-frac of the tree)")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
     ap.add_argument("--no-obj", action="store_true")
@@ -470,8 +515,8 @@ def main():
 
     W, H = args.width, args.height
     t = time.time()
-    spec = api.TerrainSpec(depth=args.depth, seed=1, octaves=4, boxes=64, box_emin=0.01, box_emax=0.1)
-    tree = api.StoredOctree.terrain(spec, 8192)
+    from rayt_amd.workloads import PLAIN_POSE, camera, plain_scene_spec
+    tree = api.StoredOctree.terrain(plain_scene_spec(args.depth), 8192)
     info = tree.info()
     log(f"scene depth {args.depth}: {info.source_nodes} nodes, {info.blocks_count} blocks, built in {time.time() - t:.1f}s")
 
@@ -489,8 +534,7 @@ def main():
     cache.InitialFillCache()
     log(f"pool filled ({info.blocks_count} blocks) in {time.time() - t:.2f}s")
 
-    cam = api.Camera(yaw=35.0, pitch=-25.0, position=(0.05, 0.75, 1.25), fov=60.0, aspect=W / H)
-    vpi = cam.view_proj_inv()
+    vpi = camera(*PLAIN_POSE, W, H)
     flags = api.L.RT_FLAG_SHADOWS
     lod = 0.0
 
@@ -546,6 +590,7 @@ def main():
         f"({rays / avg_kernel_ms / 1e3:.0f} Mrays/s), {achieved_gbs:.0f} GB/s algorithmic")
 
     img = tr.out_image()
+    moving = measure_plain_moving(ctx, cache, tr, W, H, flags, lod)
     if args.dump:
         from rayt_amd import imageio
         os.makedirs(args.dump, exist_ok=True)
@@ -554,39 +599,62 @@ def main():
     batch_shape = tuple(int(x) for x in args.batch_shape.split(",")) if args.batch_shape else None
     batched = None if args.no_batched else measure_batched(ctx, cache, args.depth, max(3, args.steps // 3),
                                                            shape=batch_shape)
-    served = None
-    if not args.no_served:
-        scenes = []
-        for k, d in enumerate(int(x) for x in args.serve_depths.split(",")):
-            if d == args.depth and all(sd != d for _, sd in scenes):
-                scenes.append((tree, d))
-                continue
-            t = time.time()
-            if d >= 13:
-                os.environ["RAYT_BUILD_LOG"] = "1"  # the builder reports its phases: minutes of work
-            sspec = api.TerrainSpec(depth=d, seed=2 + k, octaves=4, boxes=64, box_emin=0.01, box_emax=0.1)
-            scenes.append((api.StoredOctree.terrain(sspec, 8192), d))
-            log(f"served scene depth {d} (seed {2 + k}): {scenes[-1][0].info().blocks_count} blocks, "
-                f"built in {time.time() - t:.1f}s")
-        served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
-                                pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads,
-                                shape=tuple(int(x) for x in args.serve_shape.split(",")))
-        del scenes
-    outofcore = None
-    if not args.no_outofcore:
+    # Terrain scenes of the served and out-of-core runs: fBm terrain, 6 octaves, seed = depth (the out-of-core
+    # depth-14 scene is the served one). Scenes of depth >= --serve-file-depth are built into a .tree file and
+    # served from it (block reads through the file loader), the others in host RAM.
+    built = {}
+    files = []
+
+    def terrain_scene(depth: int, to_file: bool):
+        if depth in built:
+            return built[depth]
+        spec = api.TerrainSpec(depth=depth, seed=depth, octaves=6)
+        if depth >= 13:
+            os.environ["RAYT_BUILD_LOG"] = "1"  # the builder reports its progress (tens of seconds of work)
         t = time.time()
-        ospec = api.TerrainSpec(depth=args.ooc_depth, seed=14, octaves=6)
-        if args.ooc_depth >= 13:
-            os.environ["RAYT_BUILD_LOG"] = "1"  # the builder reports its phases: minutes of work
-        otree = api.StoredOctree.terrain(ospec, 8192)
-        log(f"out-of-core scene depth {args.ooc_depth} (6 octaves): {otree.info().blocks_count} blocks, "
+        need = int(1.2 * 16 * (1 << 30) * 4 ** (depth - 14))  # about 4x per level; depth 14 is ~16 GiB of blocks
+        if to_file and shutil.disk_usage(args.scene_dir).free > need:
+            path = os.path.join(args.scene_dir, f"rayt_terrain_d{depth}_s{depth}_o6.tree")
+            files.append(path)
+            tree = api.StoredOctree.terrain_file(spec, path, 8192)
+            where = f"file {path} ({os.path.getsize(path) / 2**30:.1f} GiB)"
+        else:
+            tree = api.StoredOctree.terrain(spec, 8192)
+            where = "host RAM"
+        os.environ.pop("RAYT_BUILD_LOG", None)
+        log(f"scene depth {depth} (6 octaves, seed {depth}): {tree.info().blocks_count} blocks in {where}, "
             f"built in {time.time() - t:.1f}s")
-        outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H,
-                                        pool_gb=args.ooc_pool_gb)
-        del otree
+        built[depth] = (tree, where, round(time.time() - t, 1))
+        return built[depth]
+
+    served = None
+    outofcore = None
+    try:
+        if not args.no_served:
+            depths = [int(x) for x in args.serve_depths.split(",")]
+            scenes = [(terrain_scene(d, args.serve_file_depth > 0 and d >= args.serve_file_depth)[0], d)
+                      for d in depths]
+            served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
+                                    pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads,
+                                    shape=tuple(int(x) for x in args.serve_shape.split(",")))
+            served["scenes"] = [{"depth": d, "store": built[d][1], "build_s": built[d][2]} for d in depths]
+            del scenes
+        if not args.no_outofcore:
+            otree, where, build_s = terrain_scene(args.ooc_depth, False)
+            outofcore = measure_out_of_core(ctx, otree, args.ooc_depth, args.ooc_frac, args.ooc_frames, W, H,
+                                            pool_gb=args.ooc_pool_gb)
+            outofcore["build_s"] = build_s
+            del otree
+    finally:
+        built.clear()
+        for path in files:
+            try:
+                os.remove(path)
+            except OSError:
+                pass
     obj_scene = None if args.no_obj else measure_obj_scene(ctx, args.obj_depth, args.obj_grid, args.steps, W, H)
-    cpu_plain = None if args.no_cpu_plain or args.no_cpu_baseline else \
-        measure_cpu_plain_config(ctx, max(1, min(16, os.cpu_count() or 1)))
+    host_cores = os.cpu_count() or 1
+    cpu_plain = None if args.no_cpu_plain or args.no_cpu_baseline else measure_cpu_plain_config(ctx, host_cores)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -595,7 +663,7 @@ def main():
         far = cache.far_pointers().read_all(np.uint32)
         normals = cache.channel_by_name("normals").read_all(np.uint32)
         colors = cache.channel_by_name("colors").read_all(np.uint32)
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_cores
         runs, tc = 0, 0.0
         r = None
         while tc < args.cpu_seconds or runs == 0:
@@ -606,7 +674,8 @@ def main():
             runs += 1
         cpu_rays = r["stats"][0] + r["stats"][1]
         cpu = {"value": round(cpu_rays * runs / tc / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"{runs} full {W}x{H} frame(s) of the same workload ({tc:.1f}s)"}
+               "sample": f"{runs} full {W}x{H} frame(s) of the same workload ({tc:.1f}s)",
+               "nproc": host_cores, "cpu_model": cpu_model()}
         match = bool(np.array_equal(img, r["image"]))
         log(f"CPU oracle: {cpu['value']} Mrays/s on {threads} threads; GPU frame identical to oracle: {match}")
         if not match:
@@ -636,6 +705,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "kernel_mrays_s": round(rays / avg_kernel_ms / 1e3, 1),
+        "plain_moving": moving,
         "batched": batched,
         "served": served,
         "out_of_core": outofcore,
