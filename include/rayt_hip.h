@@ -151,6 +151,12 @@ int rt_compress(rt_ctx* ctx, int n, const rt_buf* index, const rt_buf* in, rt_bu
 int rt_sort_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp);
 /* GPUScan::Scan - inclusive prefix sum of int32 in place: GPU algorithms/gpu-scan.cpp:15-43. */
 int rt_scan_i32(rt_ctx* ctx, rt_buf* data, int n, rt_buf* temp);
+/* GlobalMergeSortPassKernel(N, length, in, out): GPU algorithms/kernels/merge_sort.cl:47-73. Merges
+ * the sorted runs of `length` (a power of two) of `in` pairwise into `out` (stable). For the CLKernel
+ * shim, which runs the reference's GPUSort pass by pass. */
+int rt_merge_pass_i32(rt_ctx* ctx, int n, int length, const rt_buf* in, rt_buf* out);
+/* ScanPassKernel(N, length, in, out): GPU algorithms/kernels/scan.cl:2-12. One Hillis-Steele pass. */
+int rt_scan_pass_i32(rt_ctx* ctx, int n, int length, const rt_buf* in, rt_buf* out);
 /* No counterpart in the reference. rt_sort_i32 that allocates nothing, for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc): the same kernels and

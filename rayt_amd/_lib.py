@@ -78,6 +78,8 @@ SIGNATURES = {
     "rt_sort_i32": (I, [P, P, I, P]),
     "rt_scan_i32": (I, [P, P, I, P]),
     "rt_sort_i32_noalloc": (I, [P, P, I]),
+    "rt_merge_pass_i32": (I, [P, I, I, P, P]),
+    "rt_scan_pass_i32": (I, [P, I, I, P, P]),
     "rt_tree_open": (I, [C.c_char_p, PP]),
     "rt_tree_build_sphere": (I, [F, F, F, F, I, I, PP]),
     "rt_tree_build_terrain": (I, [C.POINTER(TerrainParams), I, PP]),

@@ -48,6 +48,46 @@ __global__ void CompressKernel(int n, const int* __restrict__ index, const int* 
     out[index[i]] = in[i];
 }
 
+// One pass of the merge sort of GPU algorithms/kernels/merge_sort.cl:47-73:
+// the input holds sorted runs of `len` elements (len a power of two); each
+// pair of runs [2m len, (2m+1) len) and [(2m+1) len, (2m+2) len) is merged
+// into one sorted run of 2 len. An element's place is its index in its run
+// plus the number of elements of the other run that precede it: strictly
+// smaller ones when it comes from the first run, smaller or equal ones when
+// it comes from the second (a binary search over the other run), so equal
+// keys keep their order.
+__global__ void MergePassKernel(int n, int len, const int* __restrict__ in, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int key = in[i];
+  const int ii = i & (len - 1);
+  const int run = i - ii;
+  const int pair = run & ~(2 * len - 1);
+  const bool first = run == pair;
+  const int other = first ? run + len : pair;  // start of the other run
+  const int other_end = min(n, other + len);
+  int lo = other, hi = max(other, other_end);  // count of other-run elements that precede `key`
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    const int v = in[mid];
+    if (first ? v < key : v <= key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  out[pair + ii + (lo - other)] = key;
+}
+
+// One Hillis-Steele pass of GPU algorithms/kernels/scan.cl:2-12: out[i] =
+// in[i] + in[i - len] where i >= len (with int32 wrap-around), else in[i].
+__global__ void ScanPassKernel(int n, int len, const int* __restrict__ in, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t v = uint32_t(in[i]);
+  if (i >= len) v += uint32_t(in[i - len]);
+  out[i] = int(v);
+}
+
 // ------------------------------------------------------------- radix sort
 // LSD radix sort of int32 keys (sign bit flipped), 8-bit digits, 4 passes.
 // Per tile of 256 x 8 keys: a digit histogram (upsweep), one scan over all
@@ -745,6 +785,33 @@ int rt_compress(rt_ctx* ctx, int n, const rt_buf* index, const rt_buf* in, rt_bu
   if (!n) return RT_OK;
   rayt::CompressKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(
       n, static_cast<const int*>(index->ptr), static_cast<const int*>(in->ptr), static_cast<int*>(out->ptr));
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_merge_pass_i32(rt_ctx* ctx, int n, int length, const rt_buf* in, rt_buf* out) {
+  RT_REQUIRE(ctx && in && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= in->size && uint64_t(n) * 4 <= out->size, RT_ERR_RANGE,
+             "n exceeds buffer");
+  RT_REQUIRE(length > 0 && (length & (length - 1)) == 0 && length <= (1 << 29), RT_ERR_RANGE,
+             "run length must be a power of two");
+  RT_REQUIRE(in->ptr != out->ptr, RT_ERR_INVALID, "a merge pass needs distinct buffers");
+  if (!n) return RT_OK;
+  rayt::MergePassKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(
+      n, length, static_cast<const int*>(in->ptr), static_cast<int*>(out->ptr));
+  RT_HIP_CHECK(hipGetLastError());
+  return RT_OK;
+}
+
+int rt_scan_pass_i32(rt_ctx* ctx, int n, int length, const rt_buf* in, rt_buf* out) {
+  RT_REQUIRE(ctx && in && out, RT_ERR_INVALID, "null argument");
+  RT_REQUIRE(n >= 0 && uint64_t(n) * 4 <= in->size && uint64_t(n) * 4 <= out->size, RT_ERR_RANGE,
+             "n exceeds buffer");
+  RT_REQUIRE(length > 0, RT_ERR_RANGE, "pass length must be positive");
+  RT_REQUIRE(in->ptr != out->ptr, RT_ERR_INVALID, "a scan pass needs distinct buffers");
+  if (!n) return RT_OK;
+  rayt::ScanPassKernel<<<dim3(rayt::Grid(n, 256)), dim3(256), 0, ctx->stream>>>(
+      n, length, static_cast<const int*>(in->ptr), static_cast<int*>(out->ptr));
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
 }
