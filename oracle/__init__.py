@@ -56,6 +56,9 @@ _SIG = {
                              U64P, I]),
     "orc_camera_vpi": (None, [F, F, F, FP, F, F, F, FP]),
     "orc_check_sphere_pool": (I, [U32P, U32P, U8P, U8P, I, F, F, F, F, I, I]),
+    "orc_traverse_check": (I, [P, U64P]),
+    "orc_check_constant_depth": (C.c_int64, [P, I, U64P]),
+    "orc_check_sphere_tree": (I, [P, F, F, F, F, I]),
     "orc_tri_box": (I, [FP, FP, FP, FP, FP]),
     "orc_linear_at": (F, [FP, I, I, F, F]),
     "orc_import_obj": (C.c_int64, [C.c_char_p, I]),
@@ -319,6 +322,26 @@ def check_sphere_pool(links, far, ch1, ch2, root, center, radius, level, faultle
     return lib().orc_check_sphere_pool(_p(links, U32P), _p(far, U32P), _p(ch1, U8P), _p(ch2, U8P), root,
                                        float(center[0]), float(center[1]), float(center[2]), float(radius), level,
                                        int(faultless))
+
+
+def traverse_check(tree: Tree):
+    """StoredOctreeTraverser + CheckTree over a .tree file: (error code, nodes reachable from the root)."""
+    n = C.c_uint64()
+    rc = lib().orc_traverse_check(tree.h, C.byref(n))
+    return rc, n.value
+
+
+def check_constant_depth(tree: Tree, expected_depth: int):
+    """DebugCheckConstantDepthTree: (depth errors, or minus the traverser's error; nodes checked)."""
+    n = C.c_uint64()
+    rc = lib().orc_check_constant_depth(tree.h, expected_depth, C.byref(n))
+    return rc, n.value
+
+
+def check_sphere_tree(tree: Tree, center, radius, level) -> int:
+    """CheckTestOctreeSphereWithLoader over a .tree file; 0 = pass."""
+    return lib().orc_check_sphere_tree(tree.h, float(center[0]), float(center[1]), float(center[2]), float(radius),
+                                       level)
 
 
 def tri_box(box_min, box_max, a, b, c) -> bool:

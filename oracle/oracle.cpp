@@ -33,6 +33,7 @@
 // reference counterpart): from hit_t - 2^-(depth+2) along the primary ray,
 // towards -normalize(2,-2,3), without LOD; it drops light 1 when it hits.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -43,6 +44,7 @@
 #include <map>
 #include <memory>
 #include <queue>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -1388,6 +1390,125 @@ void orc_camera_vpi(float yaw, float pitch, float roll, const float* pos, float 
 }
 
 // ---- sphere structure checks (stored-octree-test.cpp:39-233)
+// StoredOctreeTraverser (CPU data/stored-octree-traverser.cpp:33-120): every
+// block of a .tree file loaded, each block's roots attached to their parent
+// node in the parent block (whose stored link must equal the root's
+// parent_pointer_value), non-fault links followed inside a block, and every
+// node reachable from the root required to have its children resolved
+// (CheckTree). The reference crash()es and assert()s; the restatement
+// records the first error instead:
+//   1 a block fails to load, 2 a root's parent node is missing, 3 a parent
+//   link differs from parent_pointer_value, 4 a node is reached twice,
+//   5 a fault remains after loading every block, 6 no root block.
+struct TravNode {
+  bool resolved = false;               // children_ non-empty in the reference
+  std::array<TravNode*, 8> children{};  // null where the mask has no child
+  std::vector<uint8_t> data;            // the channels after node links
+  uint32_t initial_link = 0;
+};
+
+struct Traverser {
+  Tree* tree;
+  std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<TravNode>> nodes;
+  std::set<uint32_t> loaded;
+  TravNode* root = nullptr;
+  int error = 0;
+
+  explicit Traverser(Tree* t) : tree(t) {
+    for (uint32_t i = 0; i < t->blocks && !error; ++i) LoadIfNotLoaded(i);
+    if (!error && !root) error = 6;
+    if (!error) CheckTree(root);
+  }
+  void Fail(int e) {
+    if (!error) error = e;
+  }
+  void LoadIfNotLoaded(uint32_t index) {
+    if (error || loaded.count(index)) return;
+    Block b;
+    if (!LoadBlock(tree, index, &b)) return Fail(1);
+    if (b.parent == ROOT_BLOCK_PARENT) {
+      root = TraverseBlock(b, b.child_idx[0]);
+    } else {
+      LoadIfNotLoaded(b.parent);
+      if (error) return;
+      for (uint32_t ri = 0; ri < b.roots_count; ++ri) {
+        auto it = nodes.find({b.parent, b.parent_ptr_idx[ri]});
+        if (it == nodes.end()) return Fail(2);
+        TravNode* n = it->second.get();
+        if (n->initial_link != b.parent_ptr_value[ri]) return Fail(3);
+        n->children.fill(nullptr);
+        n->resolved = true;
+        const uint32_t mask = b.parent_ptr_value[ri] & 255;
+        int p = 0;
+        for (int i = 0; i < 8; ++i)
+          if (mask & (1u << i)) n->children[i] = TraverseBlock(b, b.child_idx[ri] + p++);
+        if (error) return;
+      }
+    }
+    loaded.insert(index);
+  }
+  TravNode* TraverseBlock(Block& b, uint32_t node_index) {
+    if (error) return nullptr;
+    if (node_index >= tree->nib) {
+      Fail(2);
+      return nullptr;
+    }
+    auto key = std::make_pair(b.index, node_index);
+    if (nodes.count(key)) {
+      Fail(4);
+      return nullptr;
+    }
+    TravNode* n = (nodes[key] = std::unique_ptr<TravNode>(new TravNode())).get();
+    size_t off = size_t(tree->nib) * 4;
+    for (size_t c = 1; c < tree->channels.size(); ++c) {
+      const int s = tree->channels[c].bytes;
+      n->data.insert(n->data.end(), &b.data[off + size_t(s) * node_index], &b.data[off + size_t(s) * (node_index + 1)]);
+      off += size_t(s) * tree->nib;
+    }
+    const uint32_t link = b.link(int(node_index));
+    n->initial_link = link;
+    const uint32_t mask = link & 255, ptr = link >> 11;
+    const bool fault = (link >> 8) & 1;
+    if (!mask) {
+      n->resolved = true;
+    } else if (!fault) {
+      n->resolved = true;
+      int p = 0;
+      for (int i = 0; i < 8; ++i)
+        if (mask & (1u << i)) n->children[i] = TraverseBlock(b, ptr + p++);
+    }
+    return n;
+  }
+  void CheckTree(TravNode* n) {
+    if (error) return;
+    if (!n->resolved) return Fail(5);
+    for (TravNode* c : n->children)
+      if (c) CheckTree(c);
+  }
+};
+
+// DebugCheckConstantDepthTree's CheckSubtree (experimental/debug-check-tree.cpp:9-35):
+// counts the nodes below `depth_left` 0 ("too deep") and the leaves above it
+// ("not deep enough").
+static void ConstantDepthRec(const TravNode* n, int depth_left, uint64_t* checked, uint64_t* errors) {
+  ++*checked;
+  if (depth_left < 0) ++*errors;
+  bool has_children = false;
+  for (const TravNode* c : n->children)
+    if (c) {
+      has_children = true;
+      ConstantDepthRec(c, depth_left - 1, checked, errors);
+    }
+  if (!has_children && depth_left > 0) ++*errors;
+}
+
+static uint64_t CountNodes(const TravNode* n) {
+  uint64_t k = 1;
+  for (const TravNode* c : n->children)
+    if (c) k += CountNodes(c);
+  return k;
+}
+
 struct SphereCheck {
   float cx, cy, cz, r2;
   int max_depth;
@@ -1480,6 +1601,83 @@ int orc_check_sphere_pool(const uint32_t* links, const uint32_t* far_ptrs, const
   float mn[3] = {0, 0, 0}, mx[3] = {1, 1, 1};
   CheckPoolRec(s, root, 0, mn, mx, 1);
   return s.error;
+}
+
+// StoredOctreeTraverser construction + CheckTree. 0 = pass (error codes at
+// Traverser); *nodes = nodes reachable from the root.
+int orc_traverse_check(void* tp, uint64_t* nodes) {
+  Traverser tr(static_cast<Tree*>(tp));
+  if (nodes) *nodes = tr.error ? 0 : CountNodes(tr.root);
+  return tr.error;
+}
+
+// DebugCheckConstantDepthTree (debug-check-tree.cpp:37-47): the traverser
+// then CheckSubtree. Returns the traverser's error as a negative number, else
+// the number of depth errors (0 = "checked ok").
+int64_t orc_check_constant_depth(void* tp, int expected_depth, uint64_t* checked) {
+  Traverser tr(static_cast<Tree*>(tp));
+  if (tr.error) return -tr.error;
+  uint64_t c = 0, errors = 0;
+  ConstantDepthRec(tr.root, expected_depth, &c, &errors);
+  if (checked) *checked = c;
+  return int64_t(errors);
+}
+
+// CheckTestOctreeSphereWithLoader (scene sources/stored-octree-test.cpp:191-210):
+// the file's channels must be node links(4), node index(4), node index %
+// 211(1); then the analytic sphere is walked against the traverser: a node
+// exactly where a cell straddles the surface, no children at the full depth,
+// and the node index channels equal to the generator's ids. 0 = pass;
+// 20 + the traverser's error; 11 wrong channels, 12 unexpected node,
+// 13 missing node, 14 children at full depth, 15 / 16 channel data differs.
+struct SphereTreeCheck {
+  SphereCheck s;
+  int error = 0;
+};
+static void CheckTreeRec(SphereTreeCheck& c, const TravNode* n, int depth, const float* minp, const float* maxp,
+                         int32_t node_id) {
+  if (c.error) return;
+  if (!Straddle(c.s, minp, maxp)) {
+    if (n) c.error = 12;
+    return;
+  }
+  if (!n) {
+    c.error = 13;
+    return;
+  }
+  if (depth < c.s.max_depth) {
+    float midp[3] = {(minp[0] + maxp[0]) * 0.5f, (minp[1] + maxp[1]) * 0.5f, (minp[2] + maxp[2]) * 0.5f};
+    for (int i = 0; i < 8 && !c.error; ++i) {
+      float nmin[3] = {minp[0], minp[1], minp[2]}, nmax[3] = {midp[0], midp[1], midp[2]};
+      if (i & 1) nmin[0] = midp[0], nmax[0] = maxp[0];
+      if (i & 2) nmin[1] = midp[1], nmax[1] = maxp[1];
+      if (i & 4) nmin[2] = midp[2], nmax[2] = maxp[2];
+      CheckTreeRec(c, n->children[i], depth + 1, nmin, nmax, int32_t(uint32_t(node_id) * 8u + uint32_t(i)));
+    }
+  } else {
+    for (const TravNode* ch : n->children)
+      if (ch) c.error = 14;
+  }
+  if (c.error) return;
+  uint32_t id;
+  memcpy(&id, n->data.data(), 4);
+  if (id != uint32_t(node_id)) c.error = 15;
+  else if (int8_t(n->data[4]) != int8_t(char(node_id % 211))) c.error = 16;
+}
+
+int orc_check_sphere_tree(void* tp, float cx, float cy, float cz, float radius, int level) {
+  Tree* t = static_cast<Tree*>(tp);
+  if (t->channels.size() != 3 || t->channels[0].name != "node links" || t->channels[0].bytes != 4 ||
+      t->channels[1].name != "node index" || t->channels[1].bytes != 4 || t->channels[2].name != "node index % 211" ||
+      t->channels[2].bytes != 1)
+    return 11;
+  Traverser tr(t);
+  if (tr.error) return 20 + tr.error;
+  SphereTreeCheck c;
+  c.s.cx = cx; c.s.cy = cy; c.s.cz = cz; c.s.r2 = radius * radius; c.s.max_depth = level;
+  float mn[3] = {0, 0, 0}, mx[3] = {1, 1, 1};
+  CheckTreeRec(c, tr.root, 0, mn, mx, 1);
+  return c.error;
 }
 
 }  // extern "C"
