@@ -81,13 +81,11 @@ uint8_t* BlockPool::StageBlock(int slot, uint32_t tree_block) {
   if (slot < 0 || slot >= max_blocks_) throw std::runtime_error("slot out of range");
   if (int(staged_slots_.size()) == staging_blocks_ && Flush() != RT_OK) throw std::runtime_error(rt_last_error());
   if (WaitStaging() != RT_OK) throw std::runtime_error(rt_last_error());
-  // Link writes into this slot issued earlier in the batch are superseded.
-  const uint64_t lo = uint64_t(slot) * nib_, hi = lo + nib_;
-  link_writes_.erase(std::remove_if(link_writes_.begin(), link_writes_.end(),
-                                    [&](const std::pair<uint64_t, uint32_t>& w) { return w.first >= lo && w.first < hi; }),
-                     link_writes_.end());
-  for (size_t i = 0; i < staged_slots_.size(); ++i)
-    if (staged_slots_[i] == slot) staged_slots_[i] = -1;  // an earlier copy in this batch is dead
+  // An earlier copy of this slot in the batch is dead, and so are the link writes into the slot issued
+  // before this copy (dropped at the flush).
+  auto it = staged_at_.find(slot);
+  if (it != staged_at_.end()) staged_slots_[it->second.first] = -1;
+  staged_at_[slot] = {staged_slots_.size(), link_writes_.size()};
   uint8_t* p = host_stage_ + uint64_t(staged_slots_.size()) * content_;
   staged_slots_.push_back(slot);
   staged_tree_blocks_.push_back(tree_block);
@@ -241,6 +239,14 @@ int BlockPool::Flush() {
     std::reverse(out.begin(), out.end());
     v.swap(out);
   };
+  if (!staged_at_.empty()) {  // writes into a slot staged after them are overwritten by the slot's copy
+    size_t k = 0;
+    for (size_t i = 0; i < link_writes_.size(); ++i) {
+      auto it = staged_at_.find(int32_t(link_writes_[i].first / uint64_t(nib_)));
+      if (it == staged_at_.end() || i >= it->second.second) link_writes_[k++] = link_writes_[i];
+    }
+    link_writes_.resize(k);
+  }
   dedupe(link_writes_);
   dedupe(far_writes_);
   const int nb = int(staged_slots_.size());
@@ -300,6 +306,7 @@ int BlockPool::Flush() {
   rc = LaunchPoolBatch(ctx_->stream, a);
   staged_slots_.clear();
   staged_tree_blocks_.clear();
+  staged_at_.clear();
   link_writes_.clear();
   far_writes_.clear();
   return rc;
@@ -421,7 +428,8 @@ int BlockCache::UnloadLRUBlock() {
   return slot;
 }
 
-int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable) {
+int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable,
+                            const std::vector<uint32_t>* children) {
   if (h.block_index >= block_to_slot_.size()) throw std::runtime_error("block index out of range");
   if (IsBlockInCache(h.block_index)) throw std::runtime_error("block already in cache");
   int slot;
@@ -433,14 +441,17 @@ int BlockCache::UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* pay
   }
   SlotInfo& info = info_[slot];
   // Enumerate child blocks (fault links); link conversion happens on the device.
-  ForEachFaultLink(reinterpret_cast<const uint32_t*>(payload), nib_, [&](int, uint32_t l) {
-    const uint32_t child = LinkPtr(l);
+  auto add_child = [&](uint32_t child) {
     if (child >= block_to_parent_slot_.size()) throw std::runtime_error("fault link outside tree");
     if (block_to_parent_slot_[child] < 0) {
       block_to_parent_slot_[child] = slot;
       info.child_blocks.push_back(int32_t(child));
     }
-  });
+  };
+  if (children)
+    for (uint32_t c : *children) add_child(c);
+  else
+    ForEachFaultLink(reinterpret_cast<const uint32_t*>(payload), nib_, [&](int, uint32_t l) { add_child(LinkPtr(l)); });
   pool_.StageBlockCopy(slot, h.block_index, payload, payload_stable);
   ++uploaded;
   if (h.parent_block_index != kRootBlockParent) {
@@ -488,6 +499,69 @@ std::vector<int> BlockCache::LruOrder() const {
   return out;
 }
 
+// ------------------------------------------------------------ ParallelJobs
+struct ParallelJobs::State {
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv, done_cv;
+  uint64_t generation = 0;
+  bool stop = false;
+  const std::function<void(size_t)>* fn = nullptr;
+  size_t n = 0;
+  std::atomic<size_t> next{0};
+  int active = 0;  // workers inside the current job
+
+  void Work() {
+    for (size_t i; (i = next.fetch_add(1)) < n;) (*fn)(i);
+  }
+  void Loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || generation != seen; });
+        if (stop) return;
+        seen = generation;
+      }
+      Work();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        --active;
+      }
+      done_cv.notify_all();
+    }
+  }
+};
+
+ParallelJobs::ParallelJobs(int workers) : st_(new State()) {
+  for (int i = 0; i < workers; ++i) st_->threads.emplace_back([s = st_.get()] { s->Loop(); });
+}
+
+ParallelJobs::~ParallelJobs() {
+  {
+    std::lock_guard<std::mutex> lk(st_->mu);
+    st_->stop = true;
+  }
+  st_->cv.notify_all();
+  for (std::thread& t : st_->threads) t.join();
+}
+
+void ParallelJobs::Run(size_t n, const std::function<void(size_t)>& fn) {
+  State& s = *st_;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.fn = &fn;
+    s.n = n;
+    s.next.store(0);
+    s.active = int(s.threads.size());
+    ++s.generation;
+  }
+  s.cv.notify_all();
+  s.Work();
+  std::unique_lock<std::mutex> lk(s.mu);
+  s.done_cv.wait(lk, [&] { return s.active == 0; });
+}
+
 // ------------------------------------------------------- BlockCacheManager
 BlockCacheManager::BlockCacheManager(rt_ctx* ctx, BlockSource* source, int max_blocks)
     : ctx_(ctx), source_(source), cache_(ctx, source->header(), max_blocks) {}
@@ -504,12 +578,12 @@ const uint8_t* BlockCacheManager::FetchBlock(uint32_t index) {
   return source_->LoadBlock(index, &scratch_) ? scratch_.data.data() : nullptr;
 }
 
-int BlockCacheManager::LoadAndUpload(uint32_t index) {
+int BlockCacheManager::LoadAndUpload(uint32_t index, const std::vector<uint32_t>* children) {
   const uint8_t* payload = FetchBlock(index);
   if (!payload) return Fail(RT_ERR_IO, "failed to load block");
   const bool its_root = scratch_.header.parent_block_index == kRootBlockParent;
   const int root_in_block = scratch_.header.roots[0].pointed_child_index;
-  const int slot = cache_.UploadBlock(scratch_.header, payload, fetched_in_place_);
+  const int slot = cache_.UploadBlock(scratch_.header, payload, fetched_in_place_, children);
   if (its_root) root_node_index_ = slot * nodes_in_block() + root_in_block;
   return RT_OK;
 }
@@ -580,21 +654,39 @@ int BlockCacheManager::RequestBlock(int block_index) {
   RT_TRY_END
 }
 
-int BlockCacheManager::EndRequestTransaction() {
+int BlockCacheManager::EndRequestTransaction(bool wait_for_device) {
   RT_REQUIRE(transaction_, RT_ERR_STATE, "no transaction in progress");
   RT_TRY_BEGIN
   // A transaction that loads nothing (a resident frame) enqueues no device work: no flush, no wait.
   const bool device_work = !transaction_blocks_.empty() || !cache_.pool().empty();
-  for (size_t i = 0; i < transaction_blocks_.size(); ++i) {
+  // The child blocks of blocks held in memory are found by scanning their links (32 KB of cold memory per
+  // 8192-node block): scanned for the whole transaction at once on several threads, then registered in
+  // load order by the loop below, as UploadBlock would have found them.
+  const size_t nt = transaction_blocks_.size();
+  const bool prescan = nt >= 8 && source_->RecordPtr(uint32_t(transaction_blocks_[0])) != nullptr;
+  if (prescan) {
+    if (!scan_) scan_.reset(new ParallelJobs(std::min(7, int(std::max(2u, std::thread::hardware_concurrency())) - 1)));
+    if (children_.size() < nt) children_.resize(nt);
+    const int nib = nodes_in_block();
+    scan_->Run(nt, [&](size_t i) {
+      std::vector<uint32_t>& kids = children_[i];
+      kids.clear();
+      const uint8_t* rec = source_->RecordPtr(uint32_t(transaction_blocks_[i]));
+      if (!rec) return;
+      ForEachFaultLink(reinterpret_cast<const uint32_t*>(rec + kBlockHeaderSize), nib,
+                       [&](int, uint32_t l) { kids.push_back(LinkPtr(l)); });
+    });
+  }
+  for (size_t i = 0; i < nt; ++i) {
     if (TransactionFilledCache()) break;
-    int rc = LoadAndUpload(uint32_t(transaction_blocks_[i]));
+    int rc = LoadAndUpload(uint32_t(transaction_blocks_[i]), prescan ? &children_[i] : nullptr);
     if (rc) return rc;
   }
   transaction_blocks_.clear();
   if (device_work) {
     int rc = cache_.pool().Flush();
     if (rc) return rc;
-    RT_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+    if (wait_for_device) RT_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
   }
   transaction_ = false;
   cache_.ReleaseMarker(transaction_marker_);

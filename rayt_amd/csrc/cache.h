@@ -13,8 +13,10 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "octree_format.h"
@@ -70,6 +72,9 @@ class BlockPool {
   std::vector<std::pair<uint64_t, uint32_t>> link_writes_;
   std::vector<std::pair<uint32_t, uint32_t>> far_writes_;
   std::vector<std::pair<uint8_t*, const uint8_t*>> pending_copies_;  // staging payload <- stable source
+  // Slots staged in this batch: the index of their (latest) staged copy, and
+  // the number of link writes issued before it (those into the slot are dead).
+  std::unordered_map<int32_t, std::pair<size_t, size_t>> staged_at_;
   bool in_flight_ = false;
   // Persistent copy threads (started by the first large batch): a batch's
   // payload copies are taken in staging order by them and the flushing thread,
@@ -109,7 +114,11 @@ class BlockCache {
   // the least recently used block when full, stages the bytes and the parent
   // pointer rewrites. Returns the slot. A payload that stays valid until the
   // pool's next Flush (payload_stable) is copied to staging at the flush.
-  int UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable = false);
+  // `children`: the block's child blocks (the pointers of its fault links, in
+  // link order) when the caller has them already; else the payload's links
+  // are scanned for them here.
+  int UploadBlock(const StoredOctreeBlockHeader& h, const uint8_t* payload, bool payload_stable = false,
+                  const std::vector<uint32_t>* children = nullptr);
   std::vector<int> LruOrder() const;
 
   uint64_t uploaded = 0, evicted = 0, patches = 0, far_writes = 0;
@@ -142,6 +151,22 @@ class BlockCache {
   int UnloadLRUBlock();
 };
 
+// A fixed set of worker threads that run the items of one job at a time
+// together with the calling thread.
+class ParallelJobs {
+ public:
+  explicit ParallelJobs(int workers);
+  ~ParallelJobs();
+  ParallelJobs(const ParallelJobs&) = delete;
+  ParallelJobs& operator=(const ParallelJobs&) = delete;
+  // Calls fn(i) for every i in [0, n) on the workers and the caller; returns when all have returned.
+  void Run(size_t n, const std::function<void(size_t)>& fn);
+
+ private:
+  struct State;
+  std::unique_ptr<State> st_;
+};
+
 class BlockCacheManager {
  public:
   BlockCacheManager(rt_ctx* ctx, BlockSource* source, int max_blocks);
@@ -160,7 +185,10 @@ class BlockCacheManager {
   int RequestBlock(int block_index);
   bool TransactionFilledCache() const { return cache_.MarkerHit(transaction_marker_); }
   bool SessionFilledCache() const { return cache_.MarkerHit(session_marker_); }
-  int EndRequestTransaction();
+  // wait_for_device = false leaves the transaction's uploads queued on the
+  // context stream (work queued after them on that stream sees them; other
+  // streams must synchronise); the reference's manager waits (WaitForAll).
+  int EndRequestTransaction(bool wait_for_device = true);
   int EndRequestSession();
   bool session_in_progress() const { return session_; }
   bool transaction_in_progress() const { return transaction_; }
@@ -175,8 +203,10 @@ class BlockCacheManager {
   std::vector<int> transaction_blocks_;
   StoredOctreeBlock scratch_;
   bool fetched_in_place_ = false;  // the last FetchBlock points into the source's memory
+  std::unique_ptr<ParallelJobs> scan_;  // child-block scans of a transaction's blocks
+  std::vector<std::vector<uint32_t>> children_;
   const uint8_t* FetchBlock(uint32_t index);
-  int LoadAndUpload(uint32_t index);
+  int LoadAndUpload(uint32_t index, const std::vector<uint32_t>* children = nullptr);
 };
 
 }  // namespace rayt

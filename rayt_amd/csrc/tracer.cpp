@@ -87,7 +87,8 @@ int Transaction(rt_tracer* t, rayt::BlockCacheManager& m, bool& done) {
   t->pass_faults.insert(t->pass_faults.end(), faults.begin(), faults.end());
   t->stats[1] += faults.size();
   const uint64_t before = m.cache().uploaded;
-  rc = m.EndRequestTransaction();
+  // The uploads stay queued: the next pass follows them on the same stream.
+  rc = m.EndRequestTransaction(false);
   if (rc) return rc;
   t->stats[2] += m.cache().uploaded - before;
   done = faults.empty() || m.SessionFilledCache();
