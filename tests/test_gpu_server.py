@@ -2,8 +2,8 @@
 feedback codes are the same alone, in a mix of scenes and sizes, under a
 framebuffer arena too small for the queue, and with a block-pool budget that
 holds one scene at a time; and all of them equal the plain single-view render
-(itself bit-exact against the oracle), which the first requests also meet
-directly against the oracle."""
+and the oracle's render of the same view. Out-of-core scenes are replayed
+request by request in the oracle's cache and compared bit for bit."""
 import numpy as np
 import pytest
 
@@ -99,17 +99,18 @@ def test_server_alone_equals_mix_equals_plain(gpu_ctx, scenes):
         np.testing.assert_array_equal(alone[i][1], mix[i][1], err_msg=f"request {i} codes alone vs mix")
         np.testing.assert_array_equal(mix[i][0], pi, err_msg=f"request {i} image vs plain")
         np.testing.assert_array_equal(mix[i][1], pc, err_msg=f"request {i} codes vs plain")
-    # the first requests of each scene directly against the oracle
-    for i in range(4):
-        vpi, w, h, s = reqs[i]
-        _, path, _, _ = scenes[s]
+    # every request directly against the oracle's render over the resident pool of its scene
+    pools = []
+    for _, path, _, _ in scenes:
         t = orc.Tree(path)
         c = orc.Cache(t, t.blocks)
         c.initial_fill()
-        r = orc.render_plain(c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
-                             c.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block, w, h, vpi, flags=SHADOWS)
-        np.testing.assert_array_equal(mix[i][0], r["image"].reshape(-1))
-        np.testing.assert_array_equal(mix[i][1], r["codes"].reshape(-1))
+        pools.append((c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
+                      c.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block))
+    for i, (vpi, w, h, s) in enumerate(reqs):
+        r = orc.render_plain(*pools[s], w, h, vpi, flags=SHADOWS)
+        np.testing.assert_array_equal(mix[i][0], r["image"].reshape(-1), err_msg=f"request {i} image vs oracle")
+        np.testing.assert_array_equal(mix[i][1], r["codes"].reshape(-1), err_msg=f"request {i} codes vs oracle")
     assert st.primary_rays == st.pixels == sum(w * h for _, w, h, _ in reqs)
     assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
 
@@ -235,26 +236,79 @@ def test_server_pinned_delivery_and_release(gpu_ctx, scenes):
 NO_HIT = 0x7FFFFFFF
 
 
+class _OraclePartialScene:
+    """The server's out-of-core scene restated over the oracle cache: each request is rendered with the
+    oracle's plain render over the pool as it stands, its codes go through the oracle's feedback
+    extraction (the server keeps one extractor, so its sampling state carries from request to request),
+    and one request session with one transaction applies them as the server does between rounds: hit
+    blocks marked used in ascending order, then duplicate hits' parents, then the faulted blocks requested
+    in descending block order until the transaction fills the pool or max_block_loads are requested
+    (gpu-ray-tracer.cpp:141-166). Every request goes alone through a round of its own here, so the
+    server's merge over the requests of a round is the identity."""
+
+    def __init__(self, path, max_blocks, max_block_loads=0):
+        self.t = orc.Tree(path)
+        self.c = orc.Cache(self.t, max_blocks)
+        self.c.initial_fill()
+        self.max_block_loads = max_block_loads
+        self.rng, self.ef = 0, 1 << 16
+
+    def request(self, vpi, w, h, flags=0):
+        c = self.c
+        r = orc.render_plain(c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
+                             c.channel_bytes(2, 4).view(np.uint32), self.t.nodes_in_block, w, h, vpi, flags=flags)
+        fb = orc.feedback_extract(r["codes"], expected_faults=self.ef, seed=self.rng)
+        self.rng, self.ef = fb["rng"], fb["expected_faults"]
+        faults = sorted(set(fb["faults"]), reverse=True)
+        if faults or fb["hits"]:
+            if self.max_block_loads:
+                faults = faults[:self.max_block_loads]
+            c.StartRequestSession()
+            c.StartRequestTransaction()
+            for b in sorted(set(fb["hits"])):
+                c.MarkBlockAsUsed(b)
+            for b in sorted(set(fb["dups"])):
+                c.MarkParentAsUsed(b)
+            for b in faults:
+                if c.TransactionFilledCache():
+                    break
+                c.RequestBlock(b)
+            c.EndRequestTransaction()
+            c.EndRequestSession()
+        return r["image"].reshape(-1), r["codes"].reshape(-1), len(set(fb["faults"]))
+
+
 def test_partial_scene_converges_to_the_resident_render(gpu_ctx, scenes):
     """A scene served from a pool of 40 % of its blocks: the first result misses blocks (their pixels show
     the coarser parent), the feedback between rounds loads them, and repeating the request converges to the
     fully resident render (same pixels; no fault codes, the same pixels hit; hit ids are pool slots).
+    Every request's image, codes and fault count equal the oracle's replay of the same requests and loads,
+    and the converged image equals the oracle's render over the fully resident pool.
     Primary rays only: a shadow ray that meets a missing block counts as unoccluded and is not fed back
     (the pixel's code is its primary ray's), in the frame path as here."""
-    tree, _, resident, depth = scenes[1]
+    tree, path, _, depth = scenes[1]
     blocks = tree.info().blocks_count
     vpi = camera_vpi(0.0, -89.0, (0.43, 0.66, 0.57), fov=30.0, aspect=1.0)
-    out, codes, _ = api.render_plain(gpu_ctx, resident, 256, 256, vpi)
-    ref_img, ref_cod = out.read_all(np.uint32), codes.read_all(np.int32)
+    t = orc.Tree(path)
+    full = orc.Cache(t, t.blocks)
+    full.initial_fill()
+    r = orc.render_plain(full.links(), full.far(), full.root(), full.channel_bytes(1, 4).view(np.uint32),
+                         full.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block, 256, 256, vpi)
+    ref_img, ref_cod = r["image"].reshape(-1), r["codes"].reshape(-1)
+    oracle = _OraclePartialScene(path, int(blocks * 0.4))
     srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20)
     try:
         s = srv.add_scene(tree, depth, max_blocks=int(blocks * 0.4))
         faults = []
-        for _ in range(16):
+        for k in range(16):
             srv.submit(vpi, 256, 256, scene=s, want_codes=True)
             srv.drain()
             (c, img, cod), = srv.poll(cap=4)
             assert c.status == 0
+            oi, oc, of = oracle.request(vpi, 256, 256)
+            np.testing.assert_array_equal(cod, oc, err_msg=f"codes of request {k}")
+            np.testing.assert_array_equal(img, oi, err_msg=f"image of request {k}")
+            assert c.fault_blocks == of, k
             faults.append(c.fault_blocks)
             assert (c.fault_blocks > 0) == bool(np.any(cod < 0))
             if c.fault_blocks == 0:
@@ -270,10 +324,15 @@ def test_partial_scene_converges_to_the_resident_render(gpu_ctx, scenes):
 
 
 def test_partial_scene_in_a_mixed_burst(gpu_ctx, scenes):
-    """A burst over a resident scene and a partial one: the resident scene's results still equal the plain
-    render, every partial-scene result is consistent with its fault count, and loads happen between rounds."""
-    tree0, _, _, d0 = scenes[0]
+    """A burst over a resident scene and a partial one: the resident scene's results still equal the
+    oracle's render, every partial-scene result is consistent with its fault count, and loads happen between rounds."""
+    tree0, path0, _, d0 = scenes[0]
     tree1, _, _, d1 = scenes[1]
+    t0 = orc.Tree(path0)
+    c0 = orc.Cache(t0, t0.blocks)
+    c0.initial_fill()
+    resident0 = (c0.links(), c0.far(), c0.root(), c0.channel_bytes(1, 4).view(np.uint32),
+                 c0.channel_bytes(2, 4).view(np.uint32), t0.nodes_in_block)
     srv = api.RenderServer(gpu_ctx, max_views_per_round=6, rounds_in_flight=2, framebuffer_bytes=64 << 20)
     try:
         s0 = srv.add_scene(tree0, d0)
@@ -287,9 +346,9 @@ def test_partial_scene_in_a_mixed_burst(gpu_ctx, scenes):
             c, img, cod = got[rid]
             assert c.status == 0
             if req[3] == s0:
-                pi, pc = _plain(gpu_ctx, scenes, (req[0], req[1], req[2], 0))
-                np.testing.assert_array_equal(img, pi)
-                np.testing.assert_array_equal(cod, pc)
+                r = orc.render_plain(*resident0, req[1], req[2], req[0], flags=SHADOWS)
+                np.testing.assert_array_equal(img, r["image"].reshape(-1))
+                np.testing.assert_array_equal(cod, r["codes"].reshape(-1))
                 assert c.fault_blocks == 0
             else:
                 assert (c.fault_blocks > 0) == bool(np.any(cod < 0))
@@ -299,19 +358,26 @@ def test_partial_scene_in_a_mixed_burst(gpu_ctx, scenes):
 
 
 def test_partial_scene_load_budget(gpu_ctx, scenes):
-    """max_block_loads bounds every load between rounds; the scene still converges, in more requests."""
-    tree, _, resident, depth = scenes[1]
+    """max_block_loads bounds every load between rounds; the scene still converges, in more requests, and
+    every request's image and codes equal the oracle's replay under the same budget."""
+    tree, path, _, depth = scenes[1]
     vpi = camera_vpi(0.0, -89.0, (0.43, 0.66, 0.57), fov=30.0, aspect=1.0)
+    max_blocks = int(tree.info().blocks_count * 0.4)
+    oracle = _OraclePartialScene(path, max_blocks, max_block_loads=3)
     srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20,
                            max_block_loads=3)
     try:
-        s = srv.add_scene(tree, depth, max_blocks=int(tree.info().blocks_count * 0.4))
+        s = srv.add_scene(tree, depth, max_blocks=max_blocks)
         faults = []
-        for _ in range(64):
-            srv.submit(vpi, 256, 256, scene=s)
+        for k in range(64):
+            srv.submit(vpi, 256, 256, scene=s, want_codes=True)
             srv.drain()
-            (c, _, _), = srv.poll(cap=4)
+            (c, img, cod), = srv.poll(cap=4)
             assert c.status == 0
+            oi, oc, of = oracle.request(vpi, 256, 256)
+            np.testing.assert_array_equal(cod, oc, err_msg=f"codes of request {k}")
+            np.testing.assert_array_equal(img, oi, err_msg=f"image of request {k}")
+            assert c.fault_blocks == of, k
             faults.append(c.fault_blocks)
             st = srv.stats()
             assert st.block_uploads <= 3 * st.load_transactions
