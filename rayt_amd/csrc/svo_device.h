@@ -258,34 +258,11 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     child = pool.link(ofs);
     if (COUNT) ++fetches;
   }
-  bool stop = (child & 255u) == 0;  // a leaf
-  if constexpr (LOD) stop = stop || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2;
-  const bool hit = fetch && stop;
-  const bool push = fetch && !hit;
-  int status = fault ? int(kFault) : bad ? int(kBadLink) : hit ? int(kHit) : int(kRunning);
-  s.child_descriptor = child;
-  if (hit) s.parent = ofs;
+  int status = fault ? int(kFault) : bad ? int(kBadLink) : int(kRunning);
 
-  if (push) {
-    const float half = s.scale_exp2 * 0.5f;
-    const float tx_center = half * s.tx_coef + tx_corner;
-    const float ty_center = half * s.ty_coef + ty_corner;
-    const float tz_center = half * s.tz_coef + tz_corner;
-    // deeper than a window-only stack holds: the ray ends as the iteration
-    // limit does (error 1), with the state the next step would report it from
-    if (tc_max < s.h && !stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)))) status = kIterLimit;
-    s.h = tc_max;
-    s.parent = ofs;
-    s.scale--;
-    s.scale_exp2 = half;
-    const bool cx = tx_center > s.t_min, cy = ty_center > s.t_min, cz = tz_center > s.t_min;
-    s.idx = (cx ? 1u : 0u) | (cy ? 2u : 0u) | (cz ? 4u : 0u);
-    s.posx = cx ? s.posx + half : s.posx;
-    s.posy = cy ? s.posy + half : s.posy;
-    s.posz = cz ? s.posz + half : s.posz;
-    s.t_max = tc_max;
-  } else if (!want) {
-    // advance
+  // The lanes that advance do not read the child word: their update comes
+  // first, while the child loads of the other lanes are in flight.
+  if (!want) {
     const bool sx = tx_corner <= tc_max, sy = ty_corner <= tc_max, sz = tz_corner <= tc_max;
     const uint32_t step_mask = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
     s.posx = sx ? s.posx - s.scale_exp2 : s.posx;
@@ -324,6 +301,34 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
         s.pending_fetch = 1;
       }
     }
+  }
+
+  bool stop = (child & 255u) == 0;  // a leaf
+  if constexpr (LOD) stop = stop || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2;
+  const bool hit = fetch && stop;
+  const bool push = fetch && !hit;
+  if (hit) status = kHit;
+  if (fetch) s.child_descriptor = child;
+  if (hit) s.parent = ofs;
+
+  if (push) {
+    const float half = s.scale_exp2 * 0.5f;
+    const float tx_center = half * s.tx_coef + tx_corner;
+    const float ty_center = half * s.ty_coef + ty_corner;
+    const float tz_center = half * s.tz_coef + tz_corner;
+    // deeper than a window-only stack holds: the ray ends as the iteration
+    // limit does (error 1), with the state the next step would report it from
+    if (tc_max < s.h && !stack.put(s.scale, make_uint2(s.parent, __float_as_uint(s.t_max)))) status = kIterLimit;
+    s.h = tc_max;
+    s.parent = ofs;
+    s.scale--;
+    s.scale_exp2 = half;
+    const bool cx = tx_center > s.t_min, cy = ty_center > s.t_min, cz = tz_center > s.t_min;
+    s.idx = (cx ? 1u : 0u) | (cy ? 2u : 0u) | (cz ? 4u : 0u);
+    s.posx = cx ? s.posx + half : s.posx;
+    s.posy = cy ? s.posy + half : s.posy;
+    s.posz = cz ? s.posz + half : s.posz;
+    s.t_max = tc_max;
   }
   return status;
 }
