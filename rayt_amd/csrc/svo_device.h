@@ -240,9 +240,13 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
 
   const uint32_t child_masks = 1u << (s.idx ^ s.octant_mask);
   const bool want = (desc & child_masks) != 0 && s.t_min <= s.t_max;
-  const bool fault = want && (desc & (1u << 8)) != 0;
+  // the fault bit (8) and the far bit (10) in one test: a far link without
+  // the fault bit reads 0x400, a near one 0, anything else is a fault
+  const uint32_t kind = desc & 0x500u;
+  const bool far_kind = kind == 0x400u, near_kind = kind == 0u;
+  const bool fault = want & !(far_kind | near_kind);
   const uint32_t ptr = desc >> 11;
-  const bool is_far = want && !fault && (desc & (1u << 10)) != 0;
+  const bool is_far = want & far_kind;
   const bool far_bad = is_far && ptr >= pool.far_count;
   uint32_t ofs = ptr + s.parent - (1u << 20);
   if (is_far && !far_bad) {
@@ -253,9 +257,10 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   // non-short-circuit forms: lane-mask ands and ors, no select through vector registers
   const bool bad = far_bad | (want & !fault & ((ofs >= pool.nodes) | (s.scale < 1)));
   const bool fetch = want && !fault && !bad;
-  uint32_t child = desc;
+  // the child word goes straight into the descriptor (the lanes that load it
+  // do not pop, and the others keep theirs or reload it in the pop below)
   if (fetch) {
-    child = pool.link(ofs);
+    s.child_descriptor = pool.link(ofs);
     if (COUNT) ++fetches;
   }
   int status = fault ? int(kFault) : bad ? int(kBadLink) : int(kRunning);
@@ -265,6 +270,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   if (!want) {
     const bool sx = tx_corner <= tc_max, sy = ty_corner <= tc_max, sz = tz_corner <= tc_max;
     const uint32_t step_mask = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
+    const uint32_t ox = __float_as_uint(s.posx), oy = __float_as_uint(s.posy), oz = __float_as_uint(s.posz);
     s.posx = sx ? s.posx - s.scale_exp2 : s.posx;
     s.posy = sy ? s.posy - s.scale_exp2 : s.posy;
     s.posz = sz ? s.posz - s.scale_exp2 : s.posz;
@@ -274,13 +280,17 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
       // pop. Highest differing bit of the stepped coordinates: the reference
       // reads it from the exponent of (float)differing_bits; the two agree
       // whenever the result is below S_MAX (the value is then exact in fp32),
-      // and both leave the loop otherwise.
-      const uint32_t dx = sx ? __float_as_uint(s.posx) ^ __float_as_uint(s.posx + s.scale_exp2) : 0u;
-      const uint32_t dy = sy ? __float_as_uint(s.posy) ^ __float_as_uint(s.posy + s.scale_exp2) : 0u;
-      const uint32_t dz = sz ? __float_as_uint(s.posz) ^ __float_as_uint(s.posz + s.scale_exp2) : 0u;
-      // dx | dy | dz is nonzero here: a stepped coordinate (below 2) changes
-      // when the cell size (at least 2^-23, scale >= 0) is added back to it
-      const int scale = 31 - __builtin_clz(dx | dy | dz);
+      // and both leave the loop otherwise. The reference differs each stepped
+      // coordinate from itself plus the cell size, which is its value before
+      // the step: the coordinates are multiples of the cell size (at least
+      // 2^-23) in [0.5, 2), where the subtraction and the addition are
+      // exact. A coordinate that did not step equals its old value and adds
+      // no bits.
+      const uint32_t diff = (__float_as_uint(s.posx) ^ ox) | (__float_as_uint(s.posy) ^ oy) |
+                            (__float_as_uint(s.posz) ^ oz);
+      // diff is nonzero here: a stepped coordinate (below 2) changes by the
+      // cell size (at least 2^-23, scale >= 0)
+      const int scale = 31 - __builtin_clz(diff);
       s.scale = scale;
       if (scale >= kSMax) {
         status = kMiss;  // the ray left the root cube: the loop test (raycast.cl:250, 254-255)
@@ -289,13 +299,15 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
         const uint2 e = stack.get(scale);
         s.parent = e.x;
         s.t_max = __uint_as_float(e.y);
-        const uint32_t shx = __float_as_uint(s.posx) >> scale;
-        const uint32_t shy = __float_as_uint(s.posy) >> scale;
-        const uint32_t shz = __float_as_uint(s.posz) >> scale;
-        s.posx = __uint_as_float(shx << scale);
-        s.posy = __uint_as_float(shy << scale);
-        s.posz = __uint_as_float(shz << scale);
-        s.idx = (shx & 1u) | ((shy & 1u) << 1) | ((shz & 1u) << 2);
+        // the coordinates cleared below the restored scale; the child index from their bits at that scale
+        const uint32_t keep = ~0u << scale;
+        const uint32_t px = __float_as_uint(s.posx) & keep, py = __float_as_uint(s.posy) & keep,
+                       pz = __float_as_uint(s.posz) & keep;
+        s.posx = __uint_as_float(px);
+        s.posy = __uint_as_float(py);
+        s.posz = __uint_as_float(pz);
+        s.idx = __builtin_amdgcn_ubfe(px, scale, 1) | (__builtin_amdgcn_ubfe(py, scale, 1) << 1) |
+                (__builtin_amdgcn_ubfe(pz, scale, 1) << 2);
         s.h = 0.0f;
         s.child_descriptor = pool.link(s.parent);
         s.pending_fetch = 1;
@@ -303,12 +315,12 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     }
   }
 
+  const uint32_t child = s.child_descriptor;  // the loaded child word on the lanes that fetched
   bool stop = (child & 255u) == 0;  // a leaf
   if constexpr (LOD) stop = stop || tc_max * s.ray_size_coef + s.ray_size_bias >= s.scale_exp2;
   const bool hit = fetch && stop;
   const bool push = fetch && !hit;
   if (hit) status = kHit;
-  if (fetch) s.child_descriptor = child;
   if (hit) s.parent = ofs;
 
   if (push) {
