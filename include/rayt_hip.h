@@ -334,7 +334,8 @@ int rt_batch_destroy(rt_batch* batch);
 int rt_batch_add_scene(rt_batch* batch, rt_cache* cache);
 /* Launch shape: tile_shape 0 = 8x8, 1 = 16x4, 2 = 32x2 pixels per wave; lds_levels (8, 10, 12, 16) deepest
  * traversal-stack levels kept in LDS; wgs_per_cu 0 = one wave per tile, dispatched by the hardware, or 1..32
- * persistent one-wave workgroups per compute unit (built for six waves per SIMD, so more than 24 wait);
+ * persistent one-wave workgroups per compute unit that take tiles from a shared queue (the LDS window caps how
+ * many are resident: 32 at 10 levels);
  * tree_depth of the batch's scenes (0..22; 0 = unknown: the
  * stack spills to a private array beyond the LDS window). With tree_depth - 1 <= lds_levels the whole stack
  * lives in LDS, under the same rule on the declared depth as rt_plain_config. Default: 8x8, 12 levels, one wave

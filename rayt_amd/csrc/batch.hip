@@ -2,26 +2,22 @@
 // resident scenes, traced by one launch (no reference counterpart; per view
 // the result is the plain single-pass frame of trace.hip, bit for bit).
 //
-// Two kernels. BatchTileKernel (wgs_per_cu == 0) runs one wave per tile of
-// the concatenated tile sequence and leaves the balancing to the hardware
-// dispatcher; it is the faster one on the measured mixes (3.6 against 4.8 ms
-// for the 64-view mix, 0.64 against 1.2 ms for a single 1080p view), as the
-// persistent kernel below pays for its queues, refills and tail. The
-// persistent kernel:
-//
-// Work layout. Every view is cut into wave tiles of 64 pixels (8x8, 16x4 or
-// 32x2); the tiles of all views form one global sequence, split into eight
-// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8). Workgroups
-// are single waves. A wave claims groups of tiles from its XCD's range with
-// one atomic and, when that range is exhausted, steals from the next XCD's;
-// after all eight ranges are exhausted every wave exits, so the grid drains.
-// Inside a wave, lanes whose ray (primary, then shadow) has finished wait
-// until kSettleMin lanes have; then all of them are completed together
-// (shading, the shadow ray's start, the stores) and the idle ones take the
-// next pixels of the wave's tiles together, so this per-ray code runs once
-// for a group of lanes. Shadow rays of one group are traced beside the
-// primary rays of the next. The traversal stack keeps the deepest
-// `lds_levels` levels in LDS.
+// Two kernels over the same tiles. Every view is cut into wave tiles of 64
+// pixels (8x8, 16x4 or 32x2) and the tiles of all views form one sequence.
+// BatchTileKernel (wgs_per_cu == 0) runs one wave per tile and leaves the
+// balancing to the hardware dispatcher. BatchKernel (wgs_per_cu > 0) is the
+// persistent form: a grid of one-wave workgroups, at most wgs_per_cu per
+// compute unit, takes the tiles from a shared queue. The queue is eight
+// queues, one per XCD (workgroup b runs on XCD b % 8): queue q holds tiles
+// q, q + 8, q + 16, ..., the tiles the hardware's round-robin dispatch gives
+// that XCD, so neighbouring tiles are traced at the same time on all XCDs
+// (contiguous eighths of the sequence ran 7 % slower). A wave claims a group
+// of its XCD's tiles with one atomic, steals from the next XCDs' queues when
+// its own is exhausted, and exits once all eight are, so the grid drains. Both kernels render a tile the same way (RenderTile): the
+// tile's view and scene are wave-uniform (scalar loads) and each lane renders
+// its pixel as the plain kernel does (RenderPixel: the primary ray, then the
+// shadow ray). The traversal stack keeps the deepest `lds_levels` levels in
+// LDS.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -38,11 +34,10 @@ namespace {
 // Tiles claimed by one atomic: up to this many, fewer when a launch has
 // too few tiles per wave for claims of this size to balance the load.
 constexpr uint32_t kMaxTilesPerGrab = 4;
-// Completion and refill run when at least this many lanes have finished
-// their ray (or when no lane is tracing), so their code is executed once
-// per group of lanes rather than once per ray. All 64 measured fastest
-// (against 16, 32 and 48) on a 64-view mix and on a single 1080p view.
-constexpr int kSettleMin = 64;
+// The eight queue counters sit 256 bytes apart, each in a memory line of
+// its own, so that the atomics on different queues do not queue up on one
+// line.
+constexpr uint32_t kCounterStride = 64;
 
 struct DevScene {
   const uint32_t* links;
@@ -69,197 +64,16 @@ struct Vec4B {
   float v[4];
 };
 
-// At most 80 VGPRs (six waves per SIMD): the traversal needs the latency hiding more than the
-// refill and shading code needs registers.
-// LOD = false: every view of the launch has lod == 0 (no LOD stop test, which
-// shadow rays never need either).
+// One wave tile of the sequence: the view holding it (a binary search over
+// the tile prefix, on wave-uniform values, so scalar loads), then one pixel per
+// lane through RenderPixel.
 template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void BatchKernel(
-    const DevView* __restrict__ views, int nviews, const DevScene* __restrict__ scenes,
-    const uint32_t* __restrict__ tile_begin, uint32_t total_tiles, uint32_t grab, uint32_t* __restrict__ counters,
-    uint32_t* __restrict__ out, int* __restrict__ codes, Vec4B bg, Lights lt, unsigned long long* __restrict__ stats) {
+__device__ __forceinline__ void RenderTile(uint32_t tile, uint32_t lane, const DevView* __restrict__ views,
+                                           int nviews, const DevScene* __restrict__ scenes,
+                                           const uint32_t* __restrict__ tile_begin, uint32_t* __restrict__ out,
+                                           int* __restrict__ codes, const Vec4B& bg, const Lights& lt, uint2* lds,
+                                           PixelCounters& cnt) {
   constexpr int TH = 64 / TW;
-  __shared__ uint2 lds[LDS_LEVELS * 64];
-  StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
-  SplitStack<LDS_LEVELS, 64, SPILL> stack;
-  stack.spill = spill.v;
-  stack.lds = lds + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-  const uint32_t xcd = blockIdx.x & 7u;
-  const uint32_t per = (total_tiles + 7u) / 8u;
-  PixelCounters cnt;
-  uint32_t fetches = 0;
-
-  // wave-uniform chunk state
-  uint32_t q_rel = 0, chunk_first = 0, chunk_pix = 0, chunk_npix = 0;
-  bool queues_done = false;
-
-  // lane state
-  bool has_ray = false;  // the lane holds a pixel
-  bool tracing = false;  // ... whose current ray needs more steps
-  int status = kRunning;
-  int phase = 0;  // 0: primary ray, 1: shadow ray
-  Trav tr;
-  PoolView pool;
-  pool.links = nullptr;
-  pool.far_ptrs = nullptr;
-  pool.nodes = pool.far_count = 0;
-  const uint32_t* normals = nullptr;
-  const uint32_t* colors = nullptr;
-  uint32_t root = 0;
-  int nib = 0;
-  uint64_t pix = 0;
-  int flags = 0, code = 0, hit_node = 0;
-  float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0;
-
-  while (true) {
-    uint64_t need = __ballot(!has_ray);
-    if (!queues_done && need) {
-      while (need && !queues_done) {
-        if (chunk_pix >= chunk_npix) {
-          while (true) {  // claim the next chunk from this XCD's queue, then steal from the others
-            if (q_rel >= 8) {
-              queues_done = true;
-              break;
-            }
-            const uint32_t q = (xcd + q_rel) & 7u;
-            const uint32_t qbegin = min(total_tiles, q * per), qend = min(total_tiles, qbegin + per);
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(&counters[q], grab);
-            t = __shfl(t, 0, 64);
-            if (t < per && qbegin + t < qend) {
-              chunk_first = qbegin + t;
-              chunk_npix = (min(qend, chunk_first + grab) - chunk_first) * 64u;
-              chunk_pix = 0;
-              break;
-            }
-            ++q_rel;
-          }
-          if (queues_done) break;
-        }
-        const uint32_t avail = chunk_npix - chunk_pix;
-        const uint32_t rank = uint32_t(__popcll(need & lanes_below));
-        const uint32_t take = min(uint32_t(__popcll(need)), avail);
-        if (!has_ray && rank < take) {
-          const uint32_t p = chunk_pix + rank;
-          const uint32_t tile = chunk_first + p / 64u, local_px = p % 64u;
-          int lo = 0, hi = nviews - 1;  // view holding the tile: last v with tile_begin[v] <= tile
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tile_begin[mid] <= tile) lo = mid; else hi = mid - 1;
-          }
-          const DevView& v = views[lo];
-          const uint32_t local = tile - tile_begin[lo];
-          const int x = int(local % uint32_t(v.tiles_x)) * TW + int(local_px % TW);
-          const int y = int(local / uint32_t(v.tiles_x)) * TH + int(local_px / TW);
-          if (x < v.width && y < v.height) {
-            const DevScene& sc = scenes[v.scene];
-            pool.links = sc.links;
-            pool.far_ptrs = sc.far_ptrs;
-            pool.nodes = sc.nodes;
-            pool.far_count = sc.far_count;
-            normals = sc.normals;
-            colors = sc.colors;
-            root = sc.root;
-            nib = sc.nodes_in_block;
-            stack.set_base(sc.lds_base);
-            const PrimaryRay r = InitRay(v.m, x, y, v.width, v.height, v.lod);
-            ox = r.origin.x, oy = r.origin.y, oz = r.origin.z;
-            dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
-            TravInit(tr, pool, root, ox, oy, oz, dx, dy, dz, r.direction.w, r.origin.w);
-            pix = v.pixel_offset + uint64_t(y) * v.width + x;
-            flags = v.flags;
-            phase = 0;
-            has_ray = tracing = true;
-            if (COUNT) ++cnt.primaries;
-          }
-        }
-        chunk_pix += take;
-        need = __ballot(!has_ray);
-      }
-    }
-    if (!__any(has_ray)) {
-      if (queues_done) break;
-      continue;  // every claimed pixel so far was outside its view; claim more
-    }
-    // Trace until kSettleMin lanes have finished (idle lanes count while
-    // pixels remain to be claimed) or no lane is tracing.
-    while (true) {
-      if (tracing) {
-        status = TravStep<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT, LOD>(tr, pool, stack, fetches);
-        tracing = status == kRunning;
-      }
-      if (!__any(tracing)) break;
-      if (__popcll(__ballot(!tracing && (has_ray || !queues_done))) >= kSettleMin) break;
-    }
-    if (!has_ray || tracing) continue;
-    CastResult res;
-    TravResult(tr, status, res);
-    uint32_t color;
-    if (phase == 0) {
-      if (res.error_code) {
-        color = PackRGBA(255, 0, 0, 255);
-        code = kNoHit;
-      } else if (res.fault_block != -1) {
-        color = ShadeNode(normals, colors, res.hit_node, lt, false);
-        code = -res.fault_block - 1;
-      } else if (res.hit_node == -1) {
-        color = 0u;
-        code = kNoHit;
-      } else {
-        code = HitCode(res.hit_node, res.hit_duplicate, nib);
-        hit_node = res.hit_node;
-        if (COUNT) ++cnt.hits;
-        bool start_shadow = false;
-        if ((flags & 1) && normals) {
-          const uint32_t n = normals[res.hit_node];
-          const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
-                      nz = float(int8_t((n >> 16) & 255u)) / 127.f;
-          start_shadow = cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f;
-        }
-        if (start_shadow) {
-          // same origin rule as ShadowOccluded
-          const float eps = __uint_as_float(uint32_t(127 - res.node_depth - 2) << 23);
-          const float tb = cl_max(0.0f, res.hit_t - eps);
-          TravInit(tr, pool, root, ox + dx * tb, oy + dy * tb, oz + dz * tb, lt.shadow_dir[0], lt.shadow_dir[1],
-                   lt.shadow_dir[2], 0.0f, 0.0f);
-          phase = 1;
-          tracing = true;
-          if (COUNT) ++cnt.shadow_rays;
-          continue;
-        }
-        color = ShadeNode(normals, colors, hit_node, lt, false);
-      }
-    } else {
-      const bool occluded = res.error_code == 0 && res.fault_block == -1 && res.hit_node != -1;
-      color = ShadeNode(normals, colors, hit_node, lt, occluded);
-    }
-    out[pix] = Blend(color, bg.v);
-    if (codes) codes[pix] = code;
-    has_ray = false;
-  }
-  if (COUNT) {
-    cnt.fetches = fetches;
-    FlushCounters(cnt, stats);
-  }
-}
-
-// One wave per tile, dispatched by the hardware (wgs_per_cu == 0): the tile
-// sequence of all views in raster order, workgroup g renders tile g and
-// exits. The wave's view and scene are uniform (scalar loads), and each lane
-// renders its pixel as the plain kernel does (RenderPixel: the primary ray,
-// then the shadow ray). The dispatcher balances the work at tile grain over
-// every XCD, and there is no queue, no refill and no tail of persistent waves.
-template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
-__global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict__ views, int nviews,
-                                                      const DevScene* __restrict__ scenes,
-                                                      const uint32_t* __restrict__ tile_begin,
-                                                      uint32_t* __restrict__ out, int* __restrict__ codes, Vec4B bg,
-                                                      Lights lt, unsigned long long* __restrict__ stats) {
-  constexpr int TH = 64 / TW;
-  __shared__ uint2 lds[LDS_LEVELS * 64];
-  const uint32_t tile = blockIdx.x, lane = threadIdx.x;
   int lo = 0, hi = nviews - 1;  // view holding the tile: last v with tile_begin[v] <= tile
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -270,7 +84,6 @@ __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict_
   const uint32_t local = tile - tile_begin[lo];
   const int x = int(local % uint32_t(v.tiles_x)) * TW + int(lane % TW);
   const int y = int(local / uint32_t(v.tiles_x)) * TH + int(lane / TW);
-  PixelCounters cnt;
   if (x < v.width && y < v.height) {
     PoolView pool;
     pool.links = sc.links;
@@ -291,6 +104,56 @@ __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict_
     out[pix] = color;
     if (codes) codes[pix] = code;
   }
+}
+
+// Persistent waves over the shared tile queue (wgs_per_cu > 0): `grab`
+// tiles per claim, from this XCD's queue first, then from the next XCDs'
+// queues. Built for eight waves per SIMD: the scalar registers of the tile
+// loop would otherwise hold the kernel to seven (3.12 against 3.32 ms on the
+// 64-view mix).
+// LOD = false: every view of the launch has lod == 0 (no LOD stop test, which
+// shadow rays never need either).
+template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void BatchKernel(
+    const DevView* __restrict__ views, int nviews, const DevScene* __restrict__ scenes,
+    const uint32_t* __restrict__ tile_begin, uint32_t total_tiles, uint32_t grab, uint32_t* __restrict__ counters,
+    uint32_t* __restrict__ out, int* __restrict__ codes, Vec4B bg, Lights lt, unsigned long long* __restrict__ stats) {
+  __shared__ uint2 lds[LDS_LEVELS * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t xcd = blockIdx.x & 7u;
+  PixelCounters cnt;
+  uint32_t q_rel = 0;
+  while (q_rel < 8u) {
+    const uint32_t q = (xcd + q_rel) & 7u;
+    const uint32_t nq = total_tiles > q ? (total_tiles - q + 7u) / 8u : 0u;  // tiles q, q + 8, q + 16, ...
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[q * kCounterStride], grab);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+    if (t >= nq) {  // this queue is exhausted: steal from the next one
+      ++q_rel;
+      continue;
+    }
+    const uint32_t last = min(nq, t + grab);
+    for (uint32_t k = t; k < last; ++k)
+      RenderTile<TW, LDS_LEVELS, COUNT, SPILL, LOD>(q + 8u * k, lane, views, nviews, scenes, tile_begin, out, codes,
+                                                    bg, lt, lds, cnt);
+  }
+  if (COUNT) FlushCounters(cnt, stats);
+}
+
+// One wave per tile, dispatched by the hardware (wgs_per_cu == 0): workgroup
+// g renders tile g and exits; there is no queue and no tail of persistent
+// waves.
+template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
+__global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict__ views, int nviews,
+                                                      const DevScene* __restrict__ scenes,
+                                                      const uint32_t* __restrict__ tile_begin,
+                                                      uint32_t* __restrict__ out, int* __restrict__ codes, Vec4B bg,
+                                                      Lights lt, unsigned long long* __restrict__ stats) {
+  __shared__ uint2 lds[LDS_LEVELS * 64];
+  PixelCounters cnt;
+  RenderTile<TW, LDS_LEVELS, COUNT, SPILL, LOD>(blockIdx.x, threadIdx.x, views, nviews, scenes, tile_begin, out,
+                                                codes, bg, lt, lds, cnt);
   if (COUNT) FlushCounters(cnt, stats);
 }
 
@@ -424,7 +287,7 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
   b->max_views = max_views;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(DevView) * max_views);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_tile_begin), 4 * (max_views + 1));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counters), 4 * 8);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counters), 4 * 8 * kCounterStride);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_stats), 8 * 4 * kStatSlots);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_scenes), sizeof(DevScene) * kMaxBatchScenes);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_views), sizeof(DevView) * max_views, 0);
@@ -500,7 +363,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   RT_HIP_CHECK(hipMemcpyAsync(b->d_tile_begin, b->h_tile_begin, 4 * (nviews + 1), hipMemcpyHostToDevice, s));
   RT_HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice,
                               s));
-  ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8);
+  ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8 * kCounterStride);
   if (count) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8 * kStatSlots);
   LaunchArgs a;
   a.stream = s;
@@ -628,8 +491,8 @@ int rt_batch_autotune(rt_batch* b, const rt_view* views, int nviews, rt_buf* out
   RT_REQUIRE(repeats >= 1 && repeats <= 16, RT_ERR_RANGE, "repeats must be in [1, 16]");
   static const int kShapes[3] = {0, 1, 2};
   static const int kLevels[4] = {8, 10, 12, 16};
-  // 0: one wave per tile; else persistent waves per CU (built for at most six per SIMD: more than 24 would wait).
-  static const int kWgs[5] = {0, 12, 16, 20, 24};
+  // 0: one wave per tile; else persistent one-wave workgroups per CU (the LDS window caps how many fit).
+  static const int kWgs[5] = {0, 8, 16, 24, 32};
   double best_t = 1e300;
   int bc[3] = {b->tile_shape, b->lds_levels, b->wgs_per_cu};
   for (int shape : kShapes)
