@@ -495,6 +495,7 @@ def main():
     ap.add_argument("--ooc-pool-gb", type=float, default=8.0, help="pool size in GiB (0: --ooc-frac of the tree)")
     ap.add_argument("--ooc-frames", type=int, default=128)
     ap.add_argument("--no-cpu-plain", action="store_true")
+    ap.add_argument("--no-plain-moving", action="store_true", help="skip the moving-camera plain frames")
     ap.add_argument("--no-obj", action="store_true")
     ap.add_argument("--obj-depth", type=int, default=11)
     ap.add_argument("--obj-grid", type=int, default=256, help="heightfield mesh of 2 * grid^2 triangles")
@@ -590,7 +591,7 @@ def main():
         f"({rays / avg_kernel_ms / 1e3:.0f} Mrays/s), {achieved_gbs:.0f} GB/s algorithmic")
 
     img = tr.out_image()
-    moving = measure_plain_moving(ctx, cache, tr, W, H, flags, lod)
+    moving = None if args.no_plain_moving else measure_plain_moving(ctx, cache, tr, W, H, flags, lod)
     if args.dump:
         from rayt_amd import imageio
         os.makedirs(args.dump, exist_ok=True)
