@@ -42,12 +42,13 @@ _SIG = {
     "orc_cache_lru_order": (I, [P, I32P, I]),
     "orc_cache_counters": (None, [P, U64P]),
     "orc_cache_op": (I, [P, I, I]),
-    "orc_render_plain": (I, [U32P, U32P, I, U32P, U32P, I, I, I, FP, F, FP, I, U32P, I32P, FP, I32P, U32P, U64P, I]),
+    "orc_render_plain": (I, [U32P, U32P, I, U32P, U32P, I, I, I, FP, F, FP, I, U32P, I32P, FP, I32P, U32P, U64P, I,
+                             C.c_uint64, C.c_uint64]),
     "orc_init_tracing_frame": (I, [FP, I, I, FP, F]),
-    "orc_raytracing_pass": (I, [I32P, FP, I, I, U32P, U32P, I, U32P, U32P, I, I, I]),
+    "orc_raytracing_pass": (I, [I32P, FP, I, I, U32P, U32P, I, U32P, U32P, I, I, I, C.c_uint64, C.c_uint64]),
     "orc_finish_tracing_frame": (I, [U32P, FP, I, I, FP, U32P, U32P]),
     "orc_ln_fixed": (None, [FP, FP, C.c_int64]),
-    "orc_raytrace_depth": (I, [U32P, U32P, I, I, I, FP, F, U32P, I32P, I]),
+    "orc_raytrace_depth": (I, [U32P, U32P, I, I, I, FP, F, U32P, I32P, I, C.c_uint64, C.c_uint64]),
     "orc_raytrace_restart": (I, [U32P, I, I, I, FP, F, U32P, I32P, I]),
     "orc_legacy_links": (I, [U32P, U32P, C.c_uint64, U32P]),
     "orc_node_to_block": (None, [I, I, I32P]),
@@ -90,6 +91,11 @@ def lib():
             fn.argtypes = args
         _lib = l
     return _lib
+
+
+def _count(a) -> int:
+    """Element count of an array argument (the traversal's bounds on the pool arrays)."""
+    return 0 if a is None else int(np.asarray(a).size)
 
 
 def _p(a, t):
@@ -229,7 +235,7 @@ def render_plain(links, far, root, normals, colors, nodes_in_block, width, heigh
     lib().orc_render_plain(_p(links, U32P), _p(far, U32P), root, _p(normals, U32P), _p(colors, U32P),
                            nodes_in_block, width, height, _p(m, FP), lod, _p(BACKGROUND, FP), flags, _p(out, U32P),
                            _p(codes, I32P), _p(hit_t, FP), _p(hit_node, I32P), _p(fetches, U32P), _p(stats, U64P),
-                           threads)
+                           threads, _count(links), _count(far))
     d = {"image": out.reshape(height, width), "codes": codes.reshape(height, width), "stats": [int(x) for x in stats]}
     if want_hits:
         d.update(hit_t=hit_t.reshape(height, width), hit_node=hit_node.reshape(height, width),
@@ -247,7 +253,8 @@ def init_tracing_frame(width, height, vpi, lod=0.0):
 def raytracing_pass(states, width, height, links, far, root, normals, colors, nodes_in_block=0, flags=0, threads=0):
     codes = np.zeros(width * height, np.int32)
     lib().orc_raytracing_pass(_p(codes, I32P), _p(states, FP), width, height, _p(links, U32P), _p(far, U32P), root,
-                              _p(normals, U32P), _p(colors, U32P), nodes_in_block, flags, threads)
+                              _p(normals, U32P), _p(colors, U32P), nodes_in_block, flags, threads, _count(links),
+                              _count(far))
     return codes
 
 
@@ -263,7 +270,7 @@ def raytrace_depth(links, far, root, width, height, vpi, lod=0.0, threads=0):
     hn = np.zeros(width * height, np.int32)
     m = np.ascontiguousarray(vpi, np.float32).reshape(16)
     lib().orc_raytrace_depth(_p(links, U32P), _p(far, U32P), root, width, height, _p(m, FP), lod, _p(out, U32P),
-                             _p(hn, I32P), threads)
+                             _p(hn, I32P), threads, _count(links), _count(far))
     return out, hn
 
 

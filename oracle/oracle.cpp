@@ -97,7 +97,14 @@ struct RayCastResult {
 };
 
 // raycast.cl:33-256. `fetches` counts node-link and far-pointer loads.
-static void CastRay(const uint32_t* node_links, const uint32_t* far_pointers, int root_node, F4 p, F4 d,
+// Extension (no reference counterpart; the product kernels do the same, and
+// on a valid pool it never triggers): a far-pointer index outside the far
+// table (far_count entries), a child offset outside the pool (node_count
+// links), or a push below the finest scale ends the ray with error code 2
+// instead of reading outside the arrays; the reference follows such links
+// unchecked.
+static void CastRay(const uint32_t* node_links, uint64_t node_count, const uint32_t* far_pointers,
+                    uint64_t far_count, int root_node, F4 p, F4 d,
                     float ray_size_coef, float ray_size_bias, RayCastResult* res, uint64_t* fetches) {
   p.x += 1.0f; p.y += 1.0f; p.z += 1.0f;  // p += (1,1,1,0)
   res->fault_block = -1;
@@ -160,13 +167,26 @@ static void CastRay(const uint32_t* node_links, const uint32_t* far_pointers, in
         return;
       }
       uint32_t ofs;
+      bool bad = false;
       if (child_descriptor & (1u << 10)) {
-        ofs = far_pointers[child_descriptor >> 11];
-        ++*fetches;
+        if ((child_descriptor >> 11) >= far_count) {
+          bad = true;
+          ofs = 0;
+        } else {
+          ofs = far_pointers[child_descriptor >> 11];
+          ++*fetches;
+        }
       } else {
         ofs = (child_descriptor >> 11) + parent - (1u << 20);
       }
       ofs += uint32_t(popcount8(child_descriptor & (child_masks - 1)));
+      if (bad || ofs >= node_count || scale < 1) {
+        res->error_code = 2;
+        res->hit_t = t_min;
+        res->hit_node = int(parent);
+        res->node_depth = S_MAX - scale - 1;
+        return;
+      }
       child_descriptor = node_links[ofs];
       ++*fetches;
       if (!(child_descriptor & 255u) || tc_max * ray_size_coef + ray_size_bias >= scale_exp2) {
@@ -305,6 +325,7 @@ static uint32_t Blend(uint32_t c, const float* bg) {
 struct Pool {
   const uint32_t* links;
   const uint32_t* far_ptrs;
+  uint64_t nodes, far_count;  // element counts of links and far_ptrs
   int root;
   const uint32_t* normals;  // may be null (treated as 0)
   const uint32_t* colors;   // may be null (treated as 0xffffffff)
@@ -322,7 +343,7 @@ static bool ShadowOccluded(const Pool& P, F4 o, F4 d, const RayCastResult& hit, 
   float tb = cl_max(0.0f, hit.hit_t - eps);
   F4 s = {o.x + d.x * tb, o.y + d.y * tb, o.z + d.z * tb, 0};
   RayCastResult r;
-  CastRay(P.links, P.far_ptrs, P.root, s, L.shadow_dir, 0.0f, 0.0f, &r, fetches);
+  CastRay(P.links, P.nodes, P.far_ptrs, P.far_count, P.root, s, L.shadow_dir, 0.0f, 0.0f, &r, fetches);
   return r.error_code == 0 && r.fault_block == -1 && r.hit_node != -1;
 }
 
@@ -351,7 +372,8 @@ static int RaytracingPassPixel(const Pool& P, TracingState& s, int nib, int flag
     return NO_HIT;
   }
   RayCastResult res;
-  CastRay(P.links, P.far_ptrs, P.root, s.ray_origin, s.ray_direction, s.ray_direction.w, s.ray_origin.w, &res,
+  CastRay(P.links, P.nodes, P.far_ptrs, P.far_count, P.root, s.ray_origin, s.ray_direction, s.ray_direction.w,
+          s.ray_origin.w, &res,
           fetches);
   if (out_res) *out_res = res;
   if (res.error_code) {
@@ -1127,8 +1149,8 @@ int orc_cache_op(void* cp, int op, int arg) {
 int orc_render_plain(const uint32_t* links, const uint32_t* far_ptrs, int root, const uint32_t* normals,
                      const uint32_t* colors, int nib, int W, int H, const float* vpi, float lod, const float* bg,
                      int flags, uint32_t* out_image, int32_t* codes, float* hit_t, int32_t* hit_node,
-                     uint32_t* fetches_per_ray, uint64_t* stats, int threads) {
-  Pool P{links, far_ptrs, root, normals, colors};
+                     uint32_t* fetches_per_ray, uint64_t* stats, int threads, uint64_t n_links, uint64_t n_far) {
+  Pool P{links, far_ptrs, n_links, n_far, root, normals, colors};
   const Lights L = MakeLights();
   int nt = Threads(threads);
   std::vector<uint64_t> st(size_t(nt) * 4, 0);
@@ -1170,8 +1192,9 @@ int orc_init_tracing_frame(float* states, int W, int H, const float* vpi, float 
   return 0;
 }
 int orc_raytracing_pass(int32_t* codes, float* states, int W, int H, const uint32_t* links, const uint32_t* far_ptrs,
-                        int root, const uint32_t* normals, const uint32_t* colors, int nib, int flags, int threads) {
-  Pool P{links, far_ptrs, root, normals, colors};
+                        int root, const uint32_t* normals, const uint32_t* colors, int nib, int flags, int threads,
+                        uint64_t n_links, uint64_t n_far) {
+  Pool P{links, far_ptrs, n_links, n_far, root, normals, colors};
   const Lights L = MakeLights();
   ParallelRows(H, Threads(threads), [&](int y, int) {
     for (int x = 0; x < W; ++x) {
@@ -1187,7 +1210,7 @@ int orc_raytracing_pass(int32_t* codes, float* states, int W, int H, const uint3
 }
 int orc_finish_tracing_frame(uint32_t* out, const float* states, int W, int H, const float* bg,
                              const uint32_t* normals, const uint32_t* colors) {
-  Pool P{nullptr, nullptr, 0, normals, colors};
+  Pool P{nullptr, nullptr, 0, 0, 0, normals, colors};
   const Lights L = MakeLights();
   for (size_t i = 0; i < size_t(W) * H; ++i) {
     TracingState s;
@@ -1204,13 +1227,14 @@ void orc_ln_fixed(const float* in, float* out, int64_t n) {
 
 // RaytraceKernel of raytracer.cl (log-depth grey). hit_node may be null.
 int orc_raytrace_depth(const uint32_t* links, const uint32_t* far_ptrs, int root, int W, int H, const float* vpi,
-                       float lod, uint32_t* out, int32_t* hit_node, int threads) {
+                       float lod, uint32_t* out, int32_t* hit_node, int threads, uint64_t n_links,
+                       uint64_t n_far) {
   ParallelRows(H, Threads(threads), [&](int y, int) {
     for (int x = 0; x < W; ++x) {
       NearFar r = NearPlaneRay(vpi, x, y, W, H, lod);
       RayCastResult res;
       uint64_t f = 0;
-      CastRay(links, far_ptrs, root, r.origin, r.dir, r.coef, r.bias, &res, &f);
+      CastRay(links, n_links, far_ptrs, n_far, root, r.origin, r.dir, r.coef, r.bias, &res, &f);
       size_t i = size_t(y) * W + x;
       if (res.error_code || res.fault_block != -1)
         out[i] = rgba(255, 0, 0, 255);
@@ -1330,7 +1354,8 @@ int orc_render_frame(void* cp, int W, int H, const float* vpi, float lod, const 
   try {
     c->StartSession();
     for (iterations = 0;; ++iterations) {
-      Pool Pl{c->links.data(), c->far_ptrs.data(), c->root_node_index, c->Channel("normals"), c->Channel("colors")};
+      Pool Pl{c->links.data(), c->far_ptrs.data(), c->links.size(), c->far_ptrs.size(), c->root_node_index,
+              c->Channel("normals"), c->Channel("colors")};
       std::vector<uint64_t> cnt(nt, 0);
       ParallelRows(H, nt, [&](int y, int t) {
         for (int x = 0; x < W; ++x) {
@@ -1369,7 +1394,8 @@ int orc_render_frame(void* cp, int W, int H, const float* vpi, float lod, const 
   } catch (...) {
     return -1;
   }
-  Pool Pl{c->links.data(), c->far_ptrs.data(), c->root_node_index, c->Channel("normals"), c->Channel("colors")};
+  Pool Pl{c->links.data(), c->far_ptrs.data(), c->links.size(), c->far_ptrs.size(), c->root_node_index,
+          c->Channel("normals"), c->Channel("colors")};
   for (size_t i = 0; i < P; ++i) out_image[i] = FinishPixel(Pl, states[i], bg, L);
   *rng_state = fb.rng;
   *expected_faults = fb.expected_faults;

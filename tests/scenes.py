@@ -51,3 +51,23 @@ def sphere_tree(center, radius, level, nodes_in_block):
     path = os.path.join(SCENE_DIR, f"sphere_{center}_{radius}_{level}_{nodes_in_block}.tree".replace(" ", ""))
     tree.write(path)
     return tree, path
+
+
+def corrupt_root_links(links, far, root):
+    """Three inner children of the root made into bad links, one of each kind the traversal checks
+    (raycast.cl follows them unchecked; here they end the ray with error 2): a near pointer past the end
+    of the pool, a far link whose far-pointer index is past the far table, and a far link whose far
+    pointer points past the pool. Returns the changed (index, word) pairs and far-table entries."""
+    desc = int(links[root])
+    assert not (desc >> 8) & 5, "the root's children must be near and loaded"
+    assert (0x1FFFFF + root - (1 << 20)) >= len(links), "the pool is too large for a near pointer past its end"
+    base = (desc >> 11) + root - (1 << 20)
+    kids = [base + i for i in range(bin(desc & 255).count("1"))]
+    inner = [k for k in kids if int(links[k]) & 255 and not (int(links[k]) >> 8) & 1][:3]
+    assert len(inner) == 3, "the root needs three inner children"
+    mask = [int(links[k]) & 255 for k in inner]
+    words = [(inner[0], (0x1FFFFF << 11) | mask[0]),                      # near pointer past the pool
+             (inner[1], ((len(far) + 5) << 11) | (1 << 10) | mask[1]),     # far index past the table
+             (inner[2], (0 << 11) | (1 << 10) | mask[2])]                  # far pointer 0 ...
+    far_words = [(0, 0x7FFFFFF0)]                                           # ... which points past the pool
+    return words, far_words

@@ -9,13 +9,16 @@ against facts they do not compute themselves:
     brute-force nearest leaf of their own rays;
   * the legacy link layout against the cache-format decode of the same pool;
   * the multi-pass out-of-core frame against the resident render: every
-    pixel that ends on a hit or a miss shows the resident colour.
+    pixel that ends on a hit or a miss shows the resident colour;
+  * the bad-link extension (a link outside the pool ends the ray with
+    error 2) against the clean pool: a ray either meets a corrupted link and
+    ends red with no feedback code, or never does and renders as before.
 """
 import numpy as np
 import pytest
 
 import oracle as orc
-from tests.scenes import POSES, camera_vpi, terrain_tree
+from tests.scenes import POSES, camera_vpi, corrupt_root_links, terrain_tree
 from tests.test_cpu_oracle import NO_HIT, _children, _leaves, _oracle_resident, _slab
 
 SHADOWS = 1
@@ -192,3 +195,41 @@ def test_out_of_core_frame_matches_resident_colours():
         passes.append(r["passes"])
     assert max(passes) >= 2, f"the pool was expected to fault and reload: passes {passes}"
     assert checked > 0.5 * 4 * W * H
+
+
+@pytest.mark.parametrize("flags", [0, SHADOWS])
+def test_bad_links_end_rays_with_error_two(flags):
+    """Three of the root's inner children get bad links (a near pointer past the pool, a far index past
+    the far table, a far pointer past the pool). Every pixel is then either the clean render's pixel (its
+    rays never reach those links) or red with NO_HIT (the traversal stops at the link instead of reading
+    outside the arrays), or, with shadow rays, a primary hit whose shadow ray met one and counts as
+    unoccluded; the red ones are many, and the counts stay consistent: fewer fetches, fewer hits."""
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    t, c = _oracle_resident(path)
+    links, far, normals, colors = _arrays(c)
+    bad_links, bad_far = links.copy(), far.copy()
+    words, far_words = corrupt_root_links(links, far, c.root())
+    for i, w in words:
+        bad_links[i] = w
+    for i, w in far_words:
+        bad_far[i] = w
+    red = 255 | (255 << 24)
+    W, H = 200, 150
+    total_red = 0
+    for yaw, pitch, pos in POSES:
+        vpi = camera_vpi(yaw, pitch, pos)
+        ok = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi, flags=flags)
+        bad = orc.render_plain(bad_links, bad_far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi,
+                               flags=flags)
+        same = (bad["image"] == ok["image"]) & (bad["codes"] == ok["codes"])
+        errored = (bad["image"] == red) & (bad["codes"] == NO_HIT)
+        # a shadow ray that meets a bad link ends in error and counts as unoccluded: the pixel keeps its
+        # code and takes its colour without the shadow (the render without shadow rays)
+        unlit = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi, flags=0)
+        lit = (bad["codes"] == ok["codes"]) & (bad["image"] == unlit["image"]) if flags else errored
+        assert np.all(same | errored | lit)
+        assert not np.any(ok["image"] == red)
+        total_red += int(np.count_nonzero(errored))
+        assert bad["stats"][2] <= ok["stats"][2] and bad["stats"][3] <= ok["stats"][3]
+    assert total_red > 1000
+

@@ -10,7 +10,7 @@ import pytest
 import oracle as orc
 from rayt_amd import api
 from tests.obj_scenes import random_textured_scene
-from tests.scenes import POSES, camera_vpi, sphere_tree, terrain_tree
+from tests.scenes import POSES, camera_vpi, corrupt_root_links, sphere_tree, terrain_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -178,6 +178,52 @@ def test_render_counts_match_oracle(gpu_ctx):
     r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, 640, 480, vpi,
                          flags=api.L.RT_FLAG_SHADOWS)
     assert st == r["stats"]
+
+
+def test_bad_links_end_rays_like_the_oracle(gpu_ctx):
+    """A pool with bad links (error code 2, a red pixel with no feedback code): the plain kernel, both
+    batched kernels and the counting kernel give the oracle's image, codes and counts."""
+    tree, path = terrain_tree(8, 8192, 8, 0)
+    m = _gpu_cache(gpu_ctx, tree)
+    t, c = _oracle_pool(path)
+    links, far, normals, colors = _oracle_arrays(t, c)
+    words, far_words = corrupt_root_links(links, far, c.root())
+    for i, w in words:
+        links[i] = w
+        m.channel(0).Write(4 * i, np.array([w], np.uint32))
+    for i, w in far_words:
+        far[i] = w
+        m.far_pointers().Write(4 * i, np.array([w], np.uint32))
+    W, H = 320, 240
+    red = 255 | (255 << 24)
+    flags = api.L.RT_FLAG_SHADOWS
+    specs = []
+    errors = 0
+    for yaw, pitch, pos in POSES:
+        vpi = camera_vpi(yaw, pitch, pos)
+        specs.append((vpi, W, H))
+        out, codes, _ = api.render_plain(gpu_ctx, m, W, H, vpi, flags=flags)
+        _, _, st = api.render_plain(gpu_ctx, m, W, H, vpi, flags=flags, count=True)
+        r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, W, H, vpi, flags=flags)
+        np.testing.assert_array_equal(codes.read_all(np.int32).reshape(H, W), r["codes"])
+        np.testing.assert_array_equal(out.read_all(np.uint32).reshape(H, W), r["image"])
+        assert st == r["stats"]
+        errors += int(np.count_nonzero(r["image"] == red))
+    assert errors > 1000, "the bad links were not reached"
+    for wgs in (0, 16):
+        b = api.BatchRenderer(m, len(specs))
+        b.config(tile_shape=0, lds_levels=10, wgs_per_cu=wgs, tree_depth=8)
+        views, total, offsets = b.make_views(specs, flags=flags)
+        out = api.Buffer(gpu_ctx, 4 * total)
+        cod = api.Buffer(gpu_ctx, 4 * total)
+        b.render(views, len(specs), out, cod)
+        img, cc = out.read_all(np.uint32), cod.read_all(np.int32)
+        for (vpi, w, h), off in zip(specs, offsets):
+            r = orc.render_plain(links, far, c.root(), normals, colors, t.nodes_in_block, w, h, vpi, flags=flags)
+            np.testing.assert_array_equal(cc[off:off + w * h], r["codes"].reshape(-1), err_msg=f"wgs {wgs}")
+            np.testing.assert_array_equal(img[off:off + w * h], r["image"].reshape(-1), err_msg=f"wgs {wgs}")
+        b.close()
+    m.close()
 
 
 # (LDS levels, declared depth) of the pass kernel's stack: spill from level 0 (the default), window-only,
