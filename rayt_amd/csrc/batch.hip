@@ -33,7 +33,7 @@ namespace {
 
 // Tiles claimed by one atomic: up to this many, fewer when a launch has
 // too few tiles per wave for claims of this size to balance the load.
-constexpr uint32_t kMaxTilesPerGrab = 4;
+constexpr uint32_t kMaxTilesPerGrab = 2;
 // The eight queue counters sit 256 bytes apart, each in a memory line of
 // its own, so that the atomics on different queues do not queue up on one
 // line.
