@@ -13,15 +13,22 @@
 - The 64-view mix over the same scene: every view, in both batched kernels
   (one wave per tile, and persistent waves over a shared tile queue), equals
   the oracle's render of that view; the mix holds a 3840x2160 view.
+- The out-of-core fly-through as measured: the depth-14 6-octave terrain
+  (161,525 blocks, streamed to a .tree file) through an 8 GiB pool, LOD 1,
+  shadow rays, the first frames of the measured spline: every frame's
+  per-pass fault lists, image and LRU order, and the pool's links and far
+  pointers after the last frame, equal the oracle's frame loop over the same
+  file (gpu-ray-tracer.cpp:66-227).
 """
 import os
+import shutil
 
 import numpy as np
 import pytest
 
 import oracle as orc
 from rayt_amd import api
-from rayt_amd.workloads import PLAIN_POSE, camera, plain_scene_spec, view_mix
+from rayt_amd.workloads import PLAIN_POSE, camera, plain_scene_spec, spline_poses, view_mix
 from tests.scenes import SCENE_DIR
 
 pytestmark = pytest.mark.gpu
@@ -132,3 +139,53 @@ def test_headline_view_mix_matches_oracle(gpu_ctx, headline, wgs_per_cu):
         exp += np.array(r["stats"], np.int64)
     assert st == [int(x) for x in exp]
     b.close()
+
+
+def test_outofcore_depth14_frames_match_oracle(gpu_ctx):
+    """The measured out-of-core configuration, frame by frame against the oracle (the first frames start
+    from the breadth-first initial fill and fault hundreds of blocks per pass)."""
+    W2, H2 = 1920, 1080
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    if shutil.disk_usage(SCENE_DIR).free < 24 * (1 << 30):
+        pytest.skip("needs 24 GiB of free disk for the depth-14 tree file")
+    path = os.path.join(SCENE_DIR, "outofcore_d14.tree")
+    try:
+        tree = api.StoredOctree.terrain_file(api.TerrainSpec(depth=14, seed=14, octaves=6), path, 8192)
+        info = tree.info()
+        assert info.blocks_count == 161525
+        max_blocks = int(8 * (1 << 30) / (info.bytes_in_block_content + 36))
+        api.plain_config(gpu_ctx, 14, 4, 14)
+        m = api.GPUOctreeCacheManager(max_blocks, tree, gpu_ctx)
+        m.InitialFillCache()
+        t = orc.Tree(path)
+        c = orc.Cache(t, max_blocks)
+        c.initial_fill()
+        assert m.root_node_index() == c.root()
+        tr = api.GPURayTracer(m, W2, H2)
+        tr.set_frame_time_limit(1e9)
+        tr.set_lod_voxel_size(1.0)
+        tr.set_flags(SHADOWS)
+        tr.set_seed(3)
+        rng, ef = 3, 1 << 16
+        faults = 0
+        for k, (yaw, pitch, pos) in enumerate(spline_poses(128, seed=11)[:4]):
+            vpi = api.Camera(yaw=yaw, pitch=pitch, position=pos, fov=60.0, aspect=W2 / H2).view_proj_inv()
+            tr.RenderFrame(vpi)
+            r = c.render_frame(W2, H2, vpi, lod=1.0, flags=SHADOWS, seed=rng, expected_faults=ef,
+                               threads=ORACLE_THREADS)
+            rng, ef = r["rng"], r["expected_faults"]
+            assert tr.pass_faults() == r["pass_faults"], f"fault lists differ in frame {k}"
+            np.testing.assert_array_equal(tr.codes(), r["codes"], err_msg=f"codes differ in frame {k}")
+            np.testing.assert_array_equal(tr.out_image(), r["image"], err_msg=f"image differs in frame {k}")
+            assert m.lru_order() == c.lru_order(), f"LRU order differs after frame {k}"
+            faults += sum(len(p) for p in r["pass_faults"])
+        assert faults > 500, faults
+        np.testing.assert_array_equal(m.channel(0).read_all(np.uint32), c.links())
+        np.testing.assert_array_equal(m.far_pointers().read_all(np.uint32), c.far())
+        tr.close()
+        m.close()
+    finally:
+        api.plain_config(gpu_ctx, 16, 4, 0)
+        if os.path.exists(path):
+            os.remove(path)
+
