@@ -45,6 +45,7 @@ _SIG = {
     "orc_render_plain": (I, [U32P, U32P, I, U32P, U32P, I, I, I, FP, F, FP, I, U32P, I32P, FP, I32P, U32P, U64P, I,
                              C.c_uint64, C.c_uint64]),
     "orc_init_tracing_frame": (I, [FP, I, I, FP, F]),
+    "orc_cache_render_plain": (I, [P, I, I, FP, F, FP, I, U32P, I32P, U64P, I]),
     "orc_raytracing_pass": (I, [I32P, FP, I, I, U32P, U32P, I, U32P, U32P, I, I, I, C.c_uint64, C.c_uint64]),
     "orc_finish_tracing_frame": (I, [U32P, FP, I, I, FP, U32P, U32P]),
     "orc_ln_fixed": (None, [FP, FP, C.c_int64]),
@@ -189,6 +190,18 @@ class Cache:
     SessionFilledCache = lambda self: bool(self.op(6))  # noqa: E731
     EndRequestTransaction = lambda self: self.op(7)  # noqa: E731
     EndRequestSession = lambda self: self.op(8)  # noqa: E731
+
+    def render_plain(self, width, height, vpi, lod=0.0, flags=0, threads=0):
+        """render_plain over this cache's pool as it stands (block-level codes), without copying the pool."""
+        out = np.zeros(width * height, np.uint32)
+        codes = np.zeros(width * height, np.int32)
+        stats = np.zeros(4, np.uint64)
+        m = np.ascontiguousarray(vpi, np.float32).reshape(16)
+        if lib().orc_cache_render_plain(self.h, width, height, _p(m, FP), lod, _p(BACKGROUND, FP), flags,
+                                        _p(out, U32P), _p(codes, I32P), _p(stats, U64P), threads) < 0:
+            raise RuntimeError("oracle render failed")
+        return {"image": out.reshape(height, width), "codes": codes.reshape(height, width),
+                "stats": [int(x) for x in stats]}
 
     def render_frame(self, width, height, vpi, lod=0.0, flags=0, max_passes=0, hits_to_extract=4000, seed=0,
                      expected_faults=1 << 16, threads=0, max_pass_faults=1 << 22):

@@ -1146,11 +1146,32 @@ int orc_cache_op(void* cp, int op, int arg) {
 // ---- single-view renders over explicit pool arrays
 // Fused InitTracingFrame + RaytracingPass + FinishTracingFrame for one pass.
 // hit_t / hit_node / fetches_per_ray may be null. stats: {primary, shadow, fetches, hits}.
+static int RenderPlainPool(const Pool& P, int nib, int W, int H, const float* vpi, float lod, const float* bg,
+                           int flags, uint32_t* out_image, int32_t* codes, float* hit_t, int32_t* hit_node,
+                           uint32_t* fetches_per_ray, uint64_t* stats, int threads);
+
 int orc_render_plain(const uint32_t* links, const uint32_t* far_ptrs, int root, const uint32_t* normals,
                      const uint32_t* colors, int nib, int W, int H, const float* vpi, float lod, const float* bg,
                      int flags, uint32_t* out_image, int32_t* codes, float* hit_t, int32_t* hit_node,
                      uint32_t* fetches_per_ray, uint64_t* stats, int threads, uint64_t n_links, uint64_t n_far) {
   Pool P{links, far_ptrs, n_links, n_far, root, normals, colors};
+  return RenderPlainPool(P, nib, W, H, vpi, lod, bg, flags, out_image, codes, hit_t, hit_node, fetches_per_ray,
+                         stats, threads);
+}
+
+// The same render over the oracle cache's pool as it stands (no copy of the pool arrays).
+int orc_cache_render_plain(void* cp, int W, int H, const float* vpi, float lod, const float* bg, int flags,
+                           uint32_t* out_image, int32_t* codes, uint64_t* stats, int threads) {
+  Cache* c = static_cast<Cache*>(cp);
+  Pool P{c->links.data(), c->far_ptrs.data(), c->links.size(), c->far_ptrs.size(), c->root_node_index,
+         c->Channel("normals"), c->Channel("colors")};
+  return RenderPlainPool(P, c->nib, W, H, vpi, lod, bg, flags, out_image, codes, nullptr, nullptr, nullptr, stats,
+                         threads);
+}
+
+static int RenderPlainPool(const Pool& P, int nib, int W, int H, const float* vpi, float lod, const float* bg,
+                           int flags, uint32_t* out_image, int32_t* codes, float* hit_t, int32_t* hit_node,
+                           uint32_t* fetches_per_ray, uint64_t* stats, int threads) {
   const Lights L = MakeLights();
   int nt = Threads(threads);
   std::vector<uint64_t> st(size_t(nt) * 4, 0);

@@ -19,6 +19,13 @@
   per-pass fault lists, image and LRU order, and the pool's links and far
   pointers after the last frame, equal the oracle's frame loop over the same
   file (gpu-ray-tracer.cpp:66-227).
+- The served scenes as measured: seeded 6-octave terrains of depths 12, 13
+  and 15 (the last streamed to a .tree file and served from it) in a 16 GiB
+  block-pool budget shared as the bench shares it (12 and 13 resident, 15
+  out of core), requests of the mix's resolutions with shadow rays, one at a
+  time: the resident scenes' results equal the oracle's render, and the
+  out-of-core scene's equal the oracle's replay of the same requests and
+  block loads (at most 128 per round), bit for bit.
 """
 import os
 import shutil
@@ -29,7 +36,7 @@ import pytest
 import oracle as orc
 from rayt_amd import api
 from rayt_amd.workloads import PLAIN_POSE, camera, plain_scene_spec, spline_poses, view_mix
-from tests.scenes import SCENE_DIR
+from tests.scenes import SCENE_DIR, OraclePartialScene
 
 pytestmark = pytest.mark.gpu
 
@@ -188,4 +195,73 @@ def test_outofcore_depth14_frames_match_oracle(gpu_ctx):
         api.plain_config(gpu_ctx, 16, 4, 0)
         if os.path.exists(path):
             os.remove(path)
+
+
+def _water_fill(trees, budget):
+    """Blocks of pool per scene under a byte budget: scenes smaller than an equal share of what is left
+    are fully resident, the others split the rest (the bench's sharing of the served pool)."""
+    sizes = [(t.info().blocks_count, t.info().bytes_in_block_content + 36) for t in trees]
+    order = sorted(range(len(trees)), key=lambda i: sizes[i][0] * sizes[i][1])
+    left, out = budget, [0] * len(trees)
+    for k, i in enumerate(order):
+        blocks, bb = sizes[i]
+        out[i] = min(blocks, (left // (len(trees) - k)) // bb)
+        left -= out[i] * bb
+    return out
+
+
+def test_served_config_requests_match_oracle(gpu_ctx):
+    """Depths 12, 13 and 15 of the measured served configuration (depth 14 is left out for disk space: its
+    path, a scene partly resident in the shared budget, is the one depth 15 takes, and the depth-14 frames
+    are checked above), through the server as measured; every request against the oracle."""
+    os.makedirs(SCENE_DIR, exist_ok=True)
+    if shutil.disk_usage(SCENE_DIR).free < 72 * (1 << 30):
+        pytest.skip("needs 72 GiB of free disk for the depth-12, 13 and 15 tree files")
+    depths = [12, 13, 15]
+    paths = [os.path.join(SCENE_DIR, f"served_d{d}.tree") for d in depths]
+    srv = None
+    try:
+        trees = [api.StoredOctree.terrain_file(api.TerrainSpec(depth=d, seed=d, octaves=6), path, 8192)
+                 for d, path in zip(depths, paths)]
+        shares = _water_fill(trees, 16 << 30)
+        partial = [sh < t.info().blocks_count for sh, t in zip(shares, trees)]
+        assert partial == [False, False, True], shares
+        srv = api.RenderServer(gpu_ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
+                               pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=0, max_block_loads=128)
+        ids = [srv.add_scene(t, d, max_blocks=sh) for t, d, sh in zip(trees, depths, shares)]
+        oracles = []
+        for path, sh, part in zip(paths, shares, partial):
+            if part:
+                oracles.append(OraclePartialScene(path, sh, max_block_loads=128))
+            else:
+                t = orc.Tree(path)
+                c = orc.Cache(t, t.blocks)
+                c.initial_fill()
+                oracles.append((t, c))
+        order = [2, 0, 2, 1, 2, 2, 0, 2, 1, 2]  # mostly the out-of-core scene, whose pool changes request by request
+        specs = view_mix(len(order), 23)
+        faults = 0
+        for k, ((vpi, w, h), s) in enumerate(zip(specs, order)):
+            srv.submit(vpi, w, h, scene=ids[s], flags=SHADOWS, want_codes=True)
+            srv.drain()
+            (c, img, cod), = srv.poll(cap=4)
+            assert c.status == 0
+            if partial[s]:
+                oi, oc, of = oracles[s].request(vpi, w, h, flags=SHADOWS)
+                assert c.fault_blocks == of, k
+                faults += of
+            else:
+                r = oracles[s][1].render_plain(w, h, vpi, flags=SHADOWS, threads=ORACLE_THREADS)
+                oi, oc = r["image"].reshape(-1), r["codes"].reshape(-1)
+                assert c.fault_blocks == 0
+            np.testing.assert_array_equal(cod, oc, err_msg=f"codes of request {k} (depth {depths[s]}, {w}x{h})")
+            np.testing.assert_array_equal(img, oi, err_msg=f"image of request {k} (depth {depths[s]}, {w}x{h})")
+        assert faults > 0, "the out-of-core scene never lacked a block"
+        assert srv.stats().load_transactions > 0
+    finally:
+        if srv is not None:
+            srv.close()
+        for path in paths:
+            if os.path.exists(path):
+                os.remove(path)
 

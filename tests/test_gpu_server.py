@@ -9,7 +9,7 @@ import pytest
 
 import oracle as orc
 from rayt_amd import api
-from tests.scenes import POSES, camera_vpi, terrain_tree
+from tests.scenes import POSES, OraclePartialScene, camera_vpi, terrain_tree
 
 pytestmark = pytest.mark.gpu
 
@@ -236,48 +236,6 @@ def test_server_pinned_delivery_and_release(gpu_ctx, scenes):
 NO_HIT = 0x7FFFFFFF
 
 
-class _OraclePartialScene:
-    """The server's out-of-core scene restated over the oracle cache: each request is rendered with the
-    oracle's plain render over the pool as it stands, its codes go through the oracle's feedback
-    extraction (the server keeps one extractor, so its sampling state carries from request to request),
-    and one request session with one transaction applies them as the server does between rounds: hit
-    blocks marked used in ascending order, then duplicate hits' parents, then the faulted blocks requested
-    in descending block order until the transaction fills the pool or max_block_loads are requested
-    (gpu-ray-tracer.cpp:141-166). Every request goes alone through a round of its own here, so the
-    server's merge over the requests of a round is the identity."""
-
-    def __init__(self, path, max_blocks, max_block_loads=0):
-        self.t = orc.Tree(path)
-        self.c = orc.Cache(self.t, max_blocks)
-        self.c.initial_fill()
-        self.max_block_loads = max_block_loads
-        self.rng, self.ef = 0, 1 << 16
-
-    def request(self, vpi, w, h, flags=0):
-        c = self.c
-        r = orc.render_plain(c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
-                             c.channel_bytes(2, 4).view(np.uint32), self.t.nodes_in_block, w, h, vpi, flags=flags)
-        fb = orc.feedback_extract(r["codes"], expected_faults=self.ef, seed=self.rng)
-        self.rng, self.ef = fb["rng"], fb["expected_faults"]
-        faults = sorted(set(fb["faults"]), reverse=True)
-        if faults or fb["hits"]:
-            if self.max_block_loads:
-                faults = faults[:self.max_block_loads]
-            c.StartRequestSession()
-            c.StartRequestTransaction()
-            for b in sorted(set(fb["hits"])):
-                c.MarkBlockAsUsed(b)
-            for b in sorted(set(fb["dups"])):
-                c.MarkParentAsUsed(b)
-            for b in faults:
-                if c.TransactionFilledCache():
-                    break
-                c.RequestBlock(b)
-            c.EndRequestTransaction()
-            c.EndRequestSession()
-        return r["image"].reshape(-1), r["codes"].reshape(-1), len(set(fb["faults"]))
-
-
 def test_partial_scene_converges_to_the_resident_render(gpu_ctx, scenes):
     """A scene served from a pool of 40 % of its blocks: the first result misses blocks (their pixels show
     the coarser parent), the feedback between rounds loads them, and repeating the request converges to the
@@ -295,7 +253,7 @@ def test_partial_scene_converges_to_the_resident_render(gpu_ctx, scenes):
     r = orc.render_plain(full.links(), full.far(), full.root(), full.channel_bytes(1, 4).view(np.uint32),
                          full.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block, 256, 256, vpi)
     ref_img, ref_cod = r["image"].reshape(-1), r["codes"].reshape(-1)
-    oracle = _OraclePartialScene(path, int(blocks * 0.4))
+    oracle = OraclePartialScene(path, int(blocks * 0.4))
     srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20)
     try:
         s = srv.add_scene(tree, depth, max_blocks=int(blocks * 0.4))
@@ -363,7 +321,7 @@ def test_partial_scene_load_budget(gpu_ctx, scenes):
     tree, path, _, depth = scenes[1]
     vpi = camera_vpi(0.0, -89.0, (0.43, 0.66, 0.57), fov=30.0, aspect=1.0)
     max_blocks = int(tree.info().blocks_count * 0.4)
-    oracle = _OraclePartialScene(path, max_blocks, max_block_loads=3)
+    oracle = OraclePartialScene(path, max_blocks, max_block_loads=3)
     srv = api.RenderServer(gpu_ctx, max_views_per_round=4, rounds_in_flight=2, framebuffer_bytes=8 << 20,
                            max_block_loads=3)
     try:
