@@ -238,8 +238,11 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
   // (posx >= 1, |coef| >= 1) and x - y rounds to +0 when x == y.
   const float tc_max = __builtin_fminf(__builtin_fminf(tx_corner, ty_corner), tz_corner);
 
-  const uint32_t child_masks = 1u << (s.idx ^ s.octant_mask);
-  const bool want = (desc & child_masks) != 0 && s.t_min <= s.t_max;
+  // The child slot's bit of the mask byte, and the mask bits below it (the
+  // children stored before this one), as bit-field extracts: the same values
+  // as (desc & (1 << slot)) and desc & ((1 << slot) - 1) & 255 for slot < 8.
+  const uint32_t slot = s.idx ^ s.octant_mask;
+  const bool want = __builtin_amdgcn_ubfe(desc, slot, 1) != 0 && s.t_min <= s.t_max;
   // the fault bit (8) and the far bit (10) in one test: a far link without
   // the fault bit reads 0x400, a near one 0, anything else is a fault
   const uint32_t kind = desc & 0x500u;
@@ -253,7 +256,7 @@ __device__ __forceinline__ int TravStep(Trav& s, const PoolView& pool, Stack& st
     ofs = pool.far(ptr);
     if (COUNT) ++fetches;
   }
-  ofs += __popc(desc & (child_masks - 1u) & 255u);
+  ofs += __popc(__builtin_amdgcn_ubfe(desc, 0, slot));
   // non-short-circuit forms: lane-mask ands and ors, no select through vector registers
   const bool bad = far_bad | (want & !fault & ((ofs >= pool.nodes) | (s.scale < 1)));
   const bool fetch = want && !fault && !bad;
