@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
 __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __restrict__ out,
                                                                 const int4* __restrict__ misc, int W, int H, Vec4 bg,
                                                                 const uint32_t* __restrict__ normals,
-                                                                const uint32_t* __restrict__ colors, Lights lt) {
+                                                                const uint32_t* __restrict__ colors, Lights lt,
+                                                                uint32_t channel_nodes) {
   const uint32_t tiles_x = (W + 15) / 16;
   int x, y;
   TilePixel(blockIdx.x, tiles_x, x, y);
@@ -152,7 +153,9 @@ __global__ __launch_bounds__(256) void FinishTracingFrameKernel(uint32_t* __rest
   const int64_t i = int64_t(y) * W + x;
   const int4 s = misc[i];
   uint32_t color = uint32_t(s.y);
-  if (uint32_t(s.z) != 0u) color = s.x >= 0 ? ShadeNode(normals, colors, s.x, lt, false) : 0u;
+  // a fault parent outside the channels (only in states no pass wrote) shades as no node
+  if (uint32_t(s.z) != 0u)
+    color = s.x >= 0 && uint32_t(s.x) < channel_nodes ? ShadeNode(normals, colors, s.x, lt, false) : 0u;
   out[i] = Blend(color, bg.v);
 }
 
@@ -418,6 +421,24 @@ int CheckTraceArgs(rt_ctx* ctx, int W, int H) {
 
 uint32_t TileGrid(int W, int H) { return uint32_t((W + 15) / 16) * uint32_t((H + 15) / 16); }
 
+// A normals or colours channel (one 4-byte word per node) must cover every
+// node of the link buffer: the shading reads them at hit node indices.
+int CheckChannels(const rt_buf* links, const rt_buf* normals, const rt_buf* colors) {
+  const uint64_t nodes = links->size / 4;
+  RT_REQUIRE(!normals || normals->size / 4 >= nodes, RT_ERR_RANGE, "normals channel smaller than the node links");
+  RT_REQUIRE(!colors || colors->size / 4 >= nodes, RT_ERR_RANGE, "colours channel smaller than the node links");
+  return RT_OK;
+}
+
+// Nodes the finishing shade may read: the smaller of the channels given (no
+// bound where neither is: the shade then reads nothing).
+uint32_t ChannelNodes(const rt_buf* normals, const rt_buf* colors) {
+  uint64_t n = ~uint64_t(0);
+  if (normals) n = std::min<uint64_t>(n, normals->size / 4);
+  if (colors) n = std::min<uint64_t>(n, colors->size / 4);
+  return uint32_t(std::min<uint64_t>(n, 0xffffffffu));
+}
+
 const uint32_t* U32(const rt_buf* b) { return b ? static_cast<const uint32_t*>(b->ptr) : nullptr; }
 
 PoolView View(const rt_buf* links, const rt_buf* far_ptrs) {
@@ -531,6 +552,8 @@ int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int
   RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
              "root node outside pool");
   RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
+  rc = CheckChannels(node_links, normals, colors);
+  if (rc) return rc;
   char* base = static_cast<char*>(states->ptr);
   const Lights lt = rayt::MakeLights();
   const bool ok = DispatchPass(ctx->plain_lds_levels, ctx->plain_spill, dim3(TileGrid(width, height)), ctx->stream,
@@ -556,7 +579,7 @@ int rt_finish_tracing_frame(rt_ctx* ctx, rt_buf* out_image, const rt_buf* states
   const char* base = static_cast<const char*>(states->ptr);
   FinishTracingFrameKernel<<<dim3(TileGrid(width, height)), dim3(256), 0, ctx->stream>>>(
       static_cast<uint32_t*>(out_image->ptr), reinterpret_cast<const int4*>(base + 32 * P), width, height, bg,
-      U32(normals), U32(colors), rayt::MakeLights());
+      U32(normals), U32(colors), rayt::MakeLights(), ChannelNodes(normals, colors));
   RT_HIP_CHECK(hipGetLastError());
   return RT_OK;
 }
@@ -657,6 +680,8 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
   RT_REQUIRE(root_node_index >= 0 && uint64_t(root_node_index) * 4 < node_links->size, RT_ERR_RANGE,
              "root node outside pool");
   RT_REQUIRE(nodes_in_block >= 0, RT_ERR_INVALID, "negative nodes_in_block");
+  rc = CheckChannels(node_links, normals, colors);
+  if (rc) return rc;
   RT_REQUIRE(!mark || (codes && nodes_in_block > 0 && !(flags & RT_FLAG_COUNT)), RT_ERR_INVALID,
              "marking needs block-level codes and the timing variant");
   RT_REQUIRE(!sched || (sched->order && sched->cost && !(flags & RT_FLAG_COUNT)), RT_ERR_INVALID,

@@ -306,3 +306,43 @@ def test_restart_kernel_matches_oracle(gpu_ctx):
                  np.float32(1.0))
         exp, _ = orc.raytrace_restart(oleg, c.root(), W, H, vpi, 1.0)
         np.testing.assert_array_equal(out.read_all(np.uint32), exp)
+
+
+def test_undersized_channels_are_refused_and_foreign_fault_parents_shade_as_no_node(gpu_ctx):
+    """The raw pass and the plain render refuse normals / colour buffers smaller than the node links (the
+    shading reads them at hit node indices); the finish kernel shades a fault parent outside the channels
+    (a state no pass wrote) as no node instead of reading past them."""
+    tree, path = terrain_tree(6, 512, 2, 2)
+    m = _gpu_cache(gpu_ctx, tree)
+    W, H = 64, 48
+    P = W * H
+    vpi = camera_vpi(*POSES[0])
+    nrm, col = m.channel_by_name("normals"), m.channel_by_name("colors")
+    short = api.Buffer(gpu_ctx, nrm.size() - 4)
+    states = api.Buffer(gpu_ctx, P * 48)
+    api.call("rt_init_tracing_frame", gpu_ctx.h, states.h, W, H, api._fp(vpi), np.float32(0.0))
+    codes, out = api.Buffer(gpu_ctx, P * 4), api.Buffer(gpu_ctx, P * 4)
+    for n, c in ((short, col), (nrm, short)):
+        with pytest.raises(api.RaytError) as e:
+            api.call("rt_raytracing_pass", gpu_ctx.h, codes.h, states.h, W, H, m.channel(0).h, m.far_pointers().h,
+                     m.root_node_index(), n.h, c.h, 0, 0)
+        assert e.value.code == api.L.RT_ERR_RANGE
+        with pytest.raises(api.RaytError) as e:
+            api.call("rt_render_plain", gpu_ctx.h, out.h, codes.h, W, H, m.channel(0).h, m.far_pointers().h,
+                     m.root_node_index(), n.h, c.h, 0, api._fp(vpi), np.float32(0.0), api._fp(api.BACKGROUND), 0,
+                     None)
+        assert e.value.code == api.L.RT_ERR_RANGE
+    # unfinished rays whose fault parent is the last node, one past it, and far beyond
+    nodes = nrm.size() // 4
+    misc = np.zeros((P, 4), np.int32)
+    misc[:, 2] = 1  # colour multiplier: unfinished
+    misc[:, 0] = np.resize(np.array([nodes - 1, nodes, 0x7ffffff0], np.int64), P).astype(np.int32)
+    states.Write(32 * P, misc.reshape(-1))
+    api.call("rt_finish_tracing_frame", gpu_ctx.h, out.h, states.h, W, H, api._fp(api.BACKGROUND), nrm.h, col.h)
+    got = out.read_all(np.uint32)
+    t, c = _oracle_pool(path)
+    _, _, normals, colors = _oracle_arrays(t, c)
+    st = np.zeros((P, 12), np.float32)
+    st.view(np.int32)[:, 8:12] = misc
+    st.view(np.int32)[:, 8][misc[:, 0] >= nodes] = -1  # the oracle's no-node fallback
+    np.testing.assert_array_equal(got, orc.finish_tracing_frame(st.reshape(-1), W, H, normals, colors))
