@@ -332,7 +332,10 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
                "bad view size");
     RT_REQUIRE(v.scene >= 0 && v.scene < int(b->scenes.size()), RT_ERR_RANGE, "view scene out of range");
     const uint64_t px = uint64_t(v.width) * v.height;
-    RT_REQUIRE((v.pixel_offset + px) * 4 <= out_image->size && (!codes || (v.pixel_offset + px) * 4 <= codes->size),
+    // in pixels, so that no sum or product can wrap
+    const uint64_t out_px = out_image->size / 4, codes_px = codes ? codes->size / 4 : out_px;
+    RT_REQUIRE(v.pixel_offset <= out_px && px <= out_px - v.pixel_offset && v.pixel_offset <= codes_px &&
+                   px <= codes_px - v.pixel_offset,
                RT_ERR_RANGE, "view outside the output buffers");
     DevView& d = b->h_views[i];
     memcpy(d.m, v.view_proj_inv, sizeof(d.m));

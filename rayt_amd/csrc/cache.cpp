@@ -62,7 +62,7 @@ BlockPool::BlockPool(rt_ctx* ctx, const StoredOctreeHeader& header, int max_bloc
 
 BlockPool::~BlockPool() {
   workers_.reset();
-  hipStreamSynchronize(ctx_->stream);
+  (void)hipStreamSynchronize(ctx_->stream);  // best effort, as a destructor
   for (rt_buf* b : channels_) rt_buf_destroy(b);
   rt_buf_destroy(far_);
   rt_buf_destroy(slot_block_);

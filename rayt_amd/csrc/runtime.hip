@@ -58,7 +58,7 @@ int RecordOut(rt_ctx* ctx, rt_evt** out) {
   }
   err = hipEventRecord(e->ev, ctx->stream);
   if (err != hipSuccess) {
-    hipEventDestroy(e->ev);
+    (void)hipEventDestroy(e->ev);
     delete e;
     return rayt::Fail(RT_ERR_HIP, std::string("hipEventRecord: ") + hipGetErrorString(err));
   }
@@ -137,8 +137,9 @@ int rt_buf_create(rt_ctx* ctx, uint64_t bytes, rt_buf** out) {
 int rt_buf_destroy(rt_buf* buf) {
   if (!buf) return RT_OK;
   if (buf->owned && buf->ptr) {
-    hipStreamSynchronize(buf->ctx->stream);
-    hipFree(buf->ptr);
+    // best effort, as a destructor: the buffer is gone whatever they report
+    (void)hipStreamSynchronize(buf->ctx->stream);
+    (void)hipFree(buf->ptr);
   }
   rayt::CtxRelease(buf->ctx);
   delete buf;
@@ -181,8 +182,9 @@ int rt_buf_read(const rt_buf* buf, uint64_t offset, uint64_t length, void* dst, 
 int rt_buf_copy(rt_buf* dst, uint64_t dst_offset, const rt_buf* src, uint64_t src_offset, uint64_t length,
                 rt_evt* const* wait, int nwait, rt_evt** out_event) {
   RT_REQUIRE(dst && src, RT_ERR_INVALID, "null argument");
-  RT_REQUIRE(length > 0 && dst_offset + length <= dst->size && src_offset + length <= src->size, RT_ERR_RANGE,
-             "copy outside buffer");
+  RT_REQUIRE(length > 0 && dst_offset <= dst->size && length <= dst->size - dst_offset && src_offset <= src->size &&
+                 length <= src->size - src_offset,
+             RT_ERR_RANGE, "copy outside buffer");
   int rc = WaitList(dst->ctx, wait, nwait);
   if (rc) return rc;
   RT_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(dst->ptr) + dst_offset,
@@ -193,7 +195,8 @@ int rt_buf_copy(rt_buf* dst, uint64_t dst_offset, const rt_buf* src, uint64_t sr
 
 int rt_buf_fill_u32(rt_buf* buf, uint64_t offset_words, uint64_t count, uint32_t value) {
   RT_REQUIRE(buf, RT_ERR_INVALID, "null buffer");
-  RT_REQUIRE((offset_words + count) * 4 <= buf->size, RT_ERR_RANGE, "fill outside buffer");
+  RT_REQUIRE(offset_words <= buf->size / 4 && count <= buf->size / 4 - offset_words, RT_ERR_RANGE,
+             "fill outside buffer");
   if (!count) return RT_OK;
   uint64_t blocks = (count + 255) / 256;
   if (blocks > 65536) blocks = 65536;
@@ -220,7 +223,7 @@ int rt_evt_retain(rt_evt* e) {
 int rt_evt_release(rt_evt* e) {
   if (!e) return RT_OK;
   if (e->refs.fetch_sub(1) == 1) {
-    hipEventDestroy(e->ev);
+    (void)hipEventDestroy(e->ev);
     delete e;
   }
   return RT_OK;

@@ -622,18 +622,21 @@ int rt_raytrace_restart(rt_ctx* ctx, rt_buf* out_image, int width, int height, c
 int rt_legacy_links_from_cache(rt_ctx* ctx, rt_buf* legacy_links, const rt_buf* node_links,
                                const rt_buf* far_pointers, uint64_t node_count) {
   RT_REQUIRE(ctx && legacy_links && node_links && far_pointers, RT_ERR_INVALID, "null argument");
-  RT_REQUIRE(node_count > 0 && node_count * 4 <= legacy_links->size && node_count * 4 <= node_links->size,
+  RT_REQUIRE(node_count > 0 && node_count <= legacy_links->size / 4 && node_count <= node_links->size / 4,
              RT_ERR_RANGE, "node count exceeds buffers");
   int* d_over = nullptr;
   RT_HIP_CHECK(hipMalloc(&d_over, sizeof(int)));
-  hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream);
+  if (const hipError_t e0 = hipMemsetAsync(d_over, 0, sizeof(int), ctx->stream); e0 != hipSuccess) {
+    (void)hipFree(d_over);
+    return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e0));
+  }
   LegacyLinksKernel<<<dim3(unsigned((node_count + 255) / 256)), dim3(256), 0, ctx->stream>>>(
       static_cast<uint32_t*>(legacy_links->ptr), U32(node_links), U32(far_pointers), far_pointers->size / 4,
       node_count, d_over);
   int over = 0;
   hipError_t e = hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  hipFree(d_over);
+  (void)hipFree(d_over);
   if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
   RT_REQUIRE(!over, RT_ERR_RANGE,
              "pool needs more than 23-bit node indices for the legacy layout, or a far link lies outside the "
@@ -698,7 +701,10 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
     unsigned long long* d_stats = nullptr;
     constexpr size_t kStatBytes = 4 * kStatSlots * sizeof(unsigned long long);
     RT_HIP_CHECK(hipMalloc(&d_stats, kStatBytes));
-    hipMemsetAsync(d_stats, 0, kStatBytes, ctx->stream);
+    if (const hipError_t e0 = hipMemsetAsync(d_stats, 0, kStatBytes, ctx->stream); e0 != hipSuccess) {
+      (void)hipFree(d_stats);
+      return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e0));
+    }
     DispatchPlain<true>(ctx->plain_waves, ctx->plain_lds_levels, ctx->plain_spill, ctx->stream,
                         static_cast<uint32_t*>(out_image->ptr),
                         codes ? static_cast<int*>(codes->ptr) : nullptr, width, height, View(node_links, far_pointers),
@@ -708,7 +714,7 @@ int rayt::RenderPlain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, 
     unsigned long long h[4 * kStatSlots] = {};
     if (e == hipSuccess) e = hipMemcpyAsync(h, d_stats, kStatBytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(d_stats);
+    (void)hipFree(d_stats);
     if (e != hipSuccess) return rayt::Fail(RT_ERR_HIP, hipGetErrorString(e));
     for (int k = 0; k < 4; ++k) {
       stats[k] = 0;
