@@ -139,14 +139,16 @@ def pool_shares(trees, budget: int):
 
 
 def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16,
-                   pool_gb: float = 16.0, max_block_loads: int = 128, shape=(0, 12, 0)):
+                   pool_gb: float = 16.0, max_block_loads: int = 128, shape=(0, 12, 0), dense_rate: float = 0.0,
+                   dense_seconds: float = 10.0):
     """Poisson request stream over several scenes through the native continuous-batching server.
 
     scenes: list of (tree, depth), sharing a block-pool budget of `pool_gb` GiB (pool_shares: scenes that
     do not fit are served out of core, loading blocks from the feedback of their requests). Each request
     is a seeded pose (position uniform in [0,1]^3, direction uniform on the sphere) and a resolution from
     the batched mix, on a uniformly drawn scene. Returns the open-loop run (p50/p99 latency at the offered
-    rate) and a closed burst (all requests queued at once: the server's throughput at saturation).
+    rate), a second open loop at `dense_rate` (0: none) and a closed burst (all requests queued at once: the
+    server's throughput at saturation).
     max_block_loads bounds the blocks loaded per out-of-core scene between rounds."""
     import numpy as np
     from rayt_amd import api
@@ -213,41 +215,47 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
             srv.release(c.id)
         return len(got)
 
-    # open loop: Poisson arrivals at `rate`
-    n = max(1, int(rate * seconds))
-    reqs = make(n)
-    gaps = rng.exponential(1.0 / rate, n)
-    srv.reset_stats()
-    missing[0] = 0
-    t0 = time.perf_counter()
-    due = t0
-    done = 0
-    next_log = t0 + 10.0
-    for i, (vpi, w, h, s) in enumerate(reqs):
-        due += gaps[i]
-        while True:
-            now = time.perf_counter()
-            if now >= due:
-                break
-            done += collect()
-            time.sleep(min(0.0005, max(0.0, due - now)))
-        srv.submit(vpi, w, h, scene=s, flags=flags, pinned=True)
-        if time.perf_counter() > next_log:
-            log(f"served: {i + 1}/{n} submitted")
-            next_log += 10.0
-    while done < n:
-        done += collect(1000)
-    st = srv.stats()
-    wall = time.perf_counter() - t0
-    open_loop = {"rate_req_s": rate, "seconds": round(wall, 2), "requests": int(st.completed),
-                 "failed": int(st.failed), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
-                 "mean_ms": round(st.mean_ms, 3), "max_ms": round(st.max_ms, 3),
-                 "mrays_s": round((st.primary_rays + st.shadow_rays) / wall / 1e6, 1),
-                 "rounds": int(st.rounds), "mean_views_per_round": round(st.mean_views_per_round, 2),
-                 "max_rounds_in_flight": int(st.max_rounds_in_flight), "scene_loads": int(st.scene_loads),
-                 "block_uploads": int(st.block_uploads), "load_transactions": int(st.load_transactions),
-                 "requests_missing_blocks": missing[0]}
-    log(f"served open loop: {open_loop}")
+    def open_loop(rate, seconds):
+        """Poisson arrivals at `rate` for `seconds`; returns the server's statistics of the run."""
+        n = max(1, int(rate * seconds))
+        reqs = make(n)
+        gaps = rng.exponential(1.0 / rate, n)
+        srv.reset_stats()
+        missing[0] = 0
+        t0 = time.perf_counter()
+        due = t0
+        done = 0
+        next_log = t0 + 10.0
+        for i, (vpi, w, h, s) in enumerate(reqs):
+            due += gaps[i]
+            while True:
+                now = time.perf_counter()
+                if now >= due:
+                    break
+                done += collect()
+                time.sleep(min(0.0005, max(0.0, due - now)))
+            srv.submit(vpi, w, h, scene=s, flags=flags, pinned=True)
+            if time.perf_counter() > next_log:
+                log(f"served at {rate:g} req/s: {i + 1}/{n} submitted")
+                next_log += 10.0
+        while done < n:
+            done += collect(1000)
+        st = srv.stats()
+        wall = time.perf_counter() - t0
+        res = {"rate_req_s": rate, "seconds": round(wall, 2), "requests": int(st.completed),
+               "failed": int(st.failed), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+               "mean_ms": round(st.mean_ms, 3), "max_ms": round(st.max_ms, 3),
+               "mrays_s": round((st.primary_rays + st.shadow_rays) / wall / 1e6, 1),
+               "rounds": int(st.rounds), "mean_views_per_round": round(st.mean_views_per_round, 2),
+               "max_rounds_in_flight": int(st.max_rounds_in_flight), "scene_loads": int(st.scene_loads),
+               "block_uploads": int(st.block_uploads), "load_transactions": int(st.load_transactions),
+               "requests_missing_blocks": missing[0]}
+        log(f"served open loop at {rate:g} req/s: {res}")
+        return res
+
+    open_stated = open_loop(rate, seconds)
+    # the same stream ten times denser: requests arrive while rounds run, so rounds batch several of them
+    open_dense = open_loop(dense_rate, dense_seconds) if dense_rate > 0 else None
 
     # closed burst: everything queued at once
     reqs = make(burst)
@@ -266,8 +274,8 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     log(f"served burst: {closed}")
     srv.close()
     return {"scene_depths": [d for _, d in scenes], "scene_pools": pools, "pool_budget_bytes": budget,
-            "framebuffer_bytes": 2 << 30, "requests_checked_bit_exact": checked, "open_loop": open_loop,
-            "burst": closed}
+            "framebuffer_bytes": 2 << 30, "requests_checked_bit_exact": checked, "open_loop": open_stated,
+            "open_loop_dense": open_dense, "burst": closed}
 
 
 def measure_plain_moving(ctx, cache, tr, W: int, H: int, flags: int, lod: float, frames: int = 64):
@@ -474,6 +482,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-dense-rate", type=float, default=2000.0,
+                    help="a second open loop at this rate (0: none), where rounds batch the requests that arrive")
+    ap.add_argument("--serve-dense-seconds", type=float, default=10.0)
     ap.add_argument("--serve-burst", type=int, default=256, help="requests queued at once for the saturation run")
     ap.add_argument("--serve-depths", default="12,13,14,15",
                     help="depths of the served scenes (fBm terrain, 6 octaves, seed = depth)")
@@ -637,7 +648,8 @@ def main():
                       for d in depths]
             served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
                                     pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads,
-                                    shape=tuple(int(x) for x in args.serve_shape.split(",")))
+                                    shape=tuple(int(x) for x in args.serve_shape.split(",")),
+                                    dense_rate=args.serve_dense_rate, dense_seconds=args.serve_dense_seconds)
             served["scenes"] = [{"depth": d, "store": built[d][1], "build_s": built[d][2]} for d in depths]
             del scenes
         if not args.no_outofcore:
