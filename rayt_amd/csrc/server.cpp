@@ -99,6 +99,11 @@ struct Scene {
   // Feedback of retired rounds not yet applied (slots and tree blocks as of
   // the pool state those rounds saw: no load happens while a round is in flight).
   std::vector<int> pend_hits, pend_dups, pend_faults;
+  // The uploads of the last load transaction stay queued on the context
+  // stream; `loaded` is recorded behind them and every round of the scene
+  // waits for it on its own stream.
+  hipEvent_t loaded = nullptr;
+  bool load_queued = false;
   void ClearPending() {
     pend_hits.clear();
     pend_dups.clear();
@@ -189,8 +194,9 @@ int rt_server::MakeResident(int id) {
       if (scenes[i].cache && scenes[i].in_flight == 0 && (victim < 0 || scenes[i].last_use < scenes[victim].last_use))
         victim = i;
     if (victim < 0) return RT_ERR_NOMEM;  // every resident scene is in use: wait for rounds to retire
-    rt_cache_destroy(scenes[victim].cache);
+    rt_cache_destroy(scenes[victim].cache);  // waits for the uploads queued on the context stream
     scenes[victim].cache = nullptr;
+    scenes[victim].load_queued = false;
     scenes[victim].ClearPending();
     pool_used -= scenes[victim].pool_bytes;
     std::lock_guard<std::mutex> g(mu);
@@ -282,6 +288,8 @@ int rt_server::Launch(Slot& s) {
   int rc = rayt::BatchSetScenes(s.batch, caches, depths);
   if (rc) return rc;
   rayt::BatchSetStream(s.batch, s.stream);
+  for (int id : s.scenes)  // the scene's last block loads, queued on the context stream
+    if (scenes[id].load_queued) RT_HIP_CHECK(hipStreamWaitEvent(s.stream, scenes[id].loaded, 0));
   rc = rayt::BatchLaunch(s.batch, views.data(), int(views.size()), arena_image, arena_codes, true);
   if (rc) return rc;
   const uint64_t cap = arena->capacity();
@@ -377,7 +385,9 @@ int rt_server::Feedback(Scene& sc, const Admitted& a, int* fault_blocks) {
 // (gpu-ray-tracer.cpp:141-166): hit blocks marked used, then the faulted
 // blocks requested in descending block order until the transaction fills the
 // pool or max_block_loads are requested (the others are reported again by
-// later requests). Runs only while no round of the scene is in flight.
+// later requests). Runs only while no round of the scene is in flight. The
+// uploads are left queued on the context stream (the scene's next rounds wait
+// for them on the device), so the server goes on retiring and launching.
 int rt_server::ApplyPending(Scene& sc) {
   auto unique = [](std::vector<int>& v, bool descending) {
     std::sort(v.begin(), v.end());
@@ -401,9 +411,17 @@ int rt_server::ApplyPending(Scene& sc) {
   if (cfg.max_block_loads > 0) nfaults = std::min(nfaults, size_t(cfg.max_block_loads));
   for (size_t i = 0; rc == RT_OK && i < nfaults && !m.TransactionFilledCache(); ++i)
     rc = m.RequestBlock(sc.pend_faults[i]);
-  if (rc == RT_OK) rc = m.EndRequestTransaction();
+  if (rc == RT_OK) rc = m.EndRequestTransaction(false);
   if (rc == RT_OK) rc = m.EndRequestSession();
-  if (rc == RT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RT_ERR_HIP;
+  if (rc == RT_OK && !sc.loaded && hipEventCreateWithFlags(&sc.loaded, hipEventDisableTiming) != hipSuccess)
+    rc = rayt::Fail(RT_ERR_HIP, "load event");
+  if (rc == RT_OK) {
+    if (hipEventRecord(sc.loaded, ctx->stream) == hipSuccess)
+      sc.load_queued = true;
+    else
+      rc = rayt::Fail(RT_ERR_HIP, "load event record");
+  }
+  if (rc != RT_OK) (void)hipStreamSynchronize(ctx->stream);  // nothing of a failed transaction is left queued
   sc.ClearPending();
   std::lock_guard<std::mutex> g(mu);
   st.block_uploads += m.cache().uploaded - before;
@@ -551,8 +569,10 @@ int rt_server_destroy(rt_server* s) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
-  for (Scene& sc : s->scenes)
+  for (Scene& sc : s->scenes) {
     if (sc.cache) rt_cache_destroy(sc.cache);
+    if (sc.loaded) (void)hipEventDestroy(sc.loaded);
+  }
   rt_buf_destroy(s->arena_image);
   rt_buf_destroy(s->arena_codes);
   if (s->pinned) (void)hipHostFree(s->pinned);
