@@ -25,6 +25,9 @@ class FeedbackExtractor {
   FeedbackExtractor& operator=(const FeedbackExtractor&) = delete;
 
   int count() const { return count_; }
+  // The stream the extraction runs on (the context's unless set); the codes
+  // it reads must be complete on the device when Process is called.
+  void set_stream(hipStream_t s) { stream_ = s ? s : ctx_->stream; }
   int hits_to_extract() const { return hits_to_extract_; }
   void set_hits_to_extract(int v) { hits_to_extract_ = v; }
   void set_seed(uint64_t s) { rng_ = s; }
@@ -57,6 +60,7 @@ class FeedbackExtractor {
 
  private:
   rt_ctx* ctx_;
+  hipStream_t stream_;
   int count_;
   int hits_to_extract_ = 4000;
   int expected_faults_ = 1 << 16;

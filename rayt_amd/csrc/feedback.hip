@@ -526,7 +526,8 @@ int ScanI32(hipStream_t s, int* data, int n) {
 }
 
 // --------------------------------------------------------- FeedbackExtractor
-FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(CtxRetain(ctx)), count_(count) {
+FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count)
+    : ctx_(CtxRetain(ctx)), stream_(ctx->stream), count_(count) {
   auto alloc = [&](void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -554,6 +555,7 @@ FeedbackExtractor::FeedbackExtractor(rt_ctx* ctx, int count) : ctx_(CtxRetain(ct
 }
 
 FeedbackExtractor::~FeedbackExtractor() {
+  (void)hipStreamSynchronize(stream_);
   (void)hipStreamSynchronize(ctx_->stream);
   (void)hipFree(fault_bits_);
   (void)hipFree(hit_bits_);
@@ -579,7 +581,7 @@ uint32_t FeedbackExtractor::NextRandom() {
 // Sorted distinct codes via the reference's own sequence of steps (sort,
 // difference flags, scan, compress), used when codes leave the bitmap domain.
 int FeedbackExtractor::UniqueBySort(int* codes, int n) {
-  hipStream_t s = ctx_->stream;
+  hipStream_t s = stream_;
   int rc = SortI32(s, codes, n, temp_);
   if (rc) return rc;
   DifferenceFlagKernel<<<dim3(Grid(n, 256)), dim3(256), 0, s>>>(n, codes, index_);
@@ -620,7 +622,7 @@ int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool bloc
 
 int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked) {
   RT_REQUIRE(n >= 0 && n <= count_, RT_ERR_RANGE, "more codes than the extractor was created for");
-  hipStream_t s = ctx_->stream;
+  hipStream_t s = stream_;
   q_codes_ = codes;
   q_n_ = n;
   q_nib_ = nodes_in_block;
@@ -664,7 +666,7 @@ int FeedbackExtractor::CompleteN() {
     unique_count_ = 0;
     return RT_OK;
   }
-  hipStream_t s = ctx_->stream;
+  hipStream_t s = stream_;
   RT_HIP_CHECK(hipEventSynchronize(copied_));
   const MarkState& hs = *reinterpret_cast<const MarkState*>(host_state_);
   if (hs.overflow) {
@@ -683,7 +685,7 @@ int FeedbackExtractor::CompleteN() {
 // the reads of growing strides until the first non-fault are replayed from the
 // known fault count, then one window of hits is read from a random start.
 int FeedbackExtractor::ReadLists() {
-  hipStream_t s = ctx_->stream;
+  hipStream_t s = stream_;
   faults_.clear();
   hits_.clear();
   dup_hits_.clear();

@@ -156,6 +156,7 @@ struct rt_server {
   uint64_t pool_used = 0;
   uint64_t use_clock = 0;
   std::unique_ptr<rayt::FeedbackExtractor> fx;  // sized for the largest request of a partial scene so far
+  hipStream_t fx_stream = nullptr;  // its stream: not the context's, where block uploads may be queued
   // shared with submitters / pollers
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -369,6 +370,7 @@ int rt_server::Feedback(Scene& sc, const Admitted& a, int* fault_blocks) {
   if (!fx || fx->count() < n) {
     fx.reset();
     fx.reset(new rayt::FeedbackExtractor(ctx, n));
+    fx->set_stream(fx_stream);
   }
   int* codes = static_cast<int*>(arena_codes->ptr) + a.px_offset;
   const int rc = fx->ProcessN(codes, n, sc.cache->mgr->nodes_in_block(), true);
@@ -527,6 +529,8 @@ int rt_server_create(rt_ctx* ctx, const rt_server_config* cfg, rt_server** out) 
   if (rc == RT_OK) rc = rt_buf_create(ctx, px * 4, &s->arena_codes);
   if (rc == RT_OK && hipHostMalloc(reinterpret_cast<void**>(&s->pinned), px * 8, 0) != hipSuccess)
     rc = rayt::Fail(RT_ERR_NOMEM, "pinned framebuffer mirror");
+  if (rc == RT_OK && hipStreamCreateWithFlags(&s->fx_stream, hipStreamNonBlocking) != hipSuccess)
+    rc = rayt::Fail(RT_ERR_HIP, "server feedback stream");
   for (int k = 0; rc == RT_OK && k < cfg->rounds_in_flight; ++k) {
     Slot& sl = s->slots[k];
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -573,6 +577,8 @@ int rt_server_destroy(rt_server* s) {
     if (sc.cache) rt_cache_destroy(sc.cache);
     if (sc.loaded) (void)hipEventDestroy(sc.loaded);
   }
+  s->fx.reset();
+  if (s->fx_stream) (void)hipStreamDestroy(s->fx_stream);
   rt_buf_destroy(s->arena_image);
   rt_buf_destroy(s->arena_codes);
   if (s->pinned) (void)hipHostFree(s->pinned);
