@@ -140,7 +140,7 @@ def pool_shares(trees, budget: int):
 
 def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: int = 9, check: int = 16,
                    pool_gb: float = 16.0, max_block_loads: int = 128, shape=(0, 12, 0), dense_rate: float = 0.0,
-                   dense_seconds: float = 10.0):
+                   dense_seconds: float = 10.0, rounds_in_flight: int = 4):
     """Poisson request stream over several scenes through the native continuous-batching server.
 
     scenes: list of (tree, depth), sharing a block-pool budget of `pool_gb` GiB (pool_shares: scenes that
@@ -156,7 +156,7 @@ def measure_served(ctx, scenes, rate: float, seconds: float, burst: int, seed: i
     flags = api.L.RT_FLAG_SHADOWS
     rng = np.random.default_rng(seed)
     budget = int(pool_gb * (1 << 30))
-    srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
+    srv = api.RenderServer(ctx, max_views_per_round=64, rounds_in_flight=rounds_in_flight, framebuffer_bytes=2 << 30,
                            pool_bytes=budget, tile_shape=shape[0], lds_levels=shape[1], wgs_per_cu=shape[2],
                            max_block_loads=max_block_loads)
     shares = pool_shares([t for t, _ in scenes], budget)
@@ -486,6 +486,7 @@ def main():
                     help="a second open loop at this rate (0: none), where rounds batch the requests that arrive")
     ap.add_argument("--serve-dense-seconds", type=float, default=10.0)
     ap.add_argument("--serve-burst", type=int, default=256, help="requests queued at once for the saturation run")
+    ap.add_argument("--serve-rounds", type=int, default=4, help="rounds the server keeps in flight (1-4)")
     ap.add_argument("--serve-depths", default="12,13,14,15",
                     help="depths of the served scenes (fBm terrain, 6 octaves, seed = depth)")
     ap.add_argument("--serve-file-depth", type=int, default=15,
@@ -649,7 +650,8 @@ def main():
             served = measure_served(ctx, scenes, args.serve_rate, args.serve_seconds, args.serve_burst,
                                     pool_gb=args.serve_pool_gb, max_block_loads=args.serve_max_loads,
                                     shape=tuple(int(x) for x in args.serve_shape.split(",")),
-                                    dense_rate=args.serve_dense_rate, dense_seconds=args.serve_dense_seconds)
+                                    dense_rate=args.serve_dense_rate, dense_seconds=args.serve_dense_seconds,
+                                    rounds_in_flight=args.serve_rounds)
             served["scenes"] = [{"depth": d, "store": built[d][1], "build_s": built[d][2]} for d in depths]
             del scenes
         if not args.no_outofcore:

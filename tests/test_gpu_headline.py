@@ -226,7 +226,7 @@ def test_served_config_requests_match_oracle(gpu_ctx):
         shares = _water_fill(trees, 16 << 30)
         partial = [sh < t.info().blocks_count for sh, t in zip(shares, trees)]
         assert partial == [False, False, True], shares
-        srv = api.RenderServer(gpu_ctx, max_views_per_round=64, rounds_in_flight=2, framebuffer_bytes=2 << 30,
+        srv = api.RenderServer(gpu_ctx, max_views_per_round=64, rounds_in_flight=4, framebuffer_bytes=2 << 30,
                                pool_bytes=16 << 30, tile_shape=0, lds_levels=12, wgs_per_cu=0, max_block_loads=128)
         ids = [srv.add_scene(t, d, max_blocks=sh) for t, d, sh in zip(trees, depths, shares)]
         oracles = []
