@@ -293,15 +293,22 @@ int rt_server::Launch(Slot& s) {
     if (scenes[id].load_queued) RT_HIP_CHECK(hipStreamWaitEvent(s.stream, scenes[id].loaded, 0));
   rc = rayt::BatchLaunch(s.batch, views.data(), int(views.size()), arena_image, arena_codes, true);
   if (rc) return rc;
+  // The results into the pinned mirror, requests adjacent in the arena in one copy each.
   const uint64_t cap = arena->capacity();
-  for (const Admitted& a : s.reqs) {
-    if (a.p.req.outputs & RT_OUT_IMAGE)
-      RT_HIP_CHECK(hipMemcpyAsync(pinned + a.px_offset * 4, static_cast<uint8_t*>(arena_image->ptr) + a.px_offset * 4,
-                                  a.px_count * 4, hipMemcpyDeviceToHost, s.stream));
-    if (a.p.req.outputs & RT_OUT_CODES)
-      RT_HIP_CHECK(hipMemcpyAsync(pinned + (cap + a.px_offset) * 4,
-                                  static_cast<uint8_t*>(arena_codes->ptr) + a.px_offset * 4, a.px_count * 4,
-                                  hipMemcpyDeviceToHost, s.stream));
+  for (int which = 0; which < 2; ++which) {
+    const int flag = which == 0 ? RT_OUT_IMAGE : RT_OUT_CODES;
+    std::vector<std::pair<uint64_t, uint64_t>> ranges;
+    for (const Admitted& a : s.reqs)
+      if (a.p.req.outputs & flag) ranges.push_back({a.px_offset, a.px_count});
+    std::sort(ranges.begin(), ranges.end());
+    const uint8_t* src = static_cast<const uint8_t*>(which == 0 ? arena_image->ptr : arena_codes->ptr);
+    uint8_t* dst = pinned + (which == 0 ? 0 : cap * 4);
+    for (size_t i = 0; i < ranges.size();) {
+      const uint64_t begin = ranges[i].first;
+      uint64_t end = begin + ranges[i].second;
+      for (++i; i < ranges.size() && ranges[i].first == end; ++i) end += ranges[i].second;
+      RT_HIP_CHECK(hipMemcpyAsync(dst + begin * 4, src + begin * 4, (end - begin) * 4, hipMemcpyDeviceToHost, s.stream));
+    }
   }
   RT_HIP_CHECK(hipEventRecord(s.done, s.stream));
   return RT_OK;
