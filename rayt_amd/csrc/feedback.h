@@ -45,12 +45,19 @@ class FeedbackExtractor {
   // later device work: EnqueueN issues the marking, the walk and the copy of
   // its result; CompleteN waits for that copy and reads the lists. The codes
   // must not change in between (the sort path may read them).
-  int EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked);
+  //
+  // Premarked walks may alternate between two sets of bitmaps (`set` 0 or
+  // 1), so that a renderer marks the next frame's codes into one set while
+  // the walk of the previous frame reads and clears the other; such a walk
+  // runs on `walk_stream` (the extractor's own stream when null) after the
+  // event `after` (the end of the trace that marked its set).
+  int EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked, int set = 0,
+               hipStream_t walk_stream = nullptr, hipEvent_t after = nullptr);
   int CompleteN();
-  // Where a renderer marks codes for the next premarked Process: the bitmaps,
-  // with the bounds of the codes the pool can produce (tree blocks for
-  // faults, 2 * slots + 2 for hit codes).
-  MarkTargets Targets(uint32_t fault_limit, uint32_t hit_limit);
+  // Where a renderer marks codes for the next premarked Process: the bitmaps
+  // of `set`, with the bounds of the codes the pool can produce (tree blocks
+  // for faults, 2 * slots + 2 for hit codes).
+  MarkTargets Targets(uint32_t fault_limit, uint32_t hit_limit, int set = 0);
 
   const std::vector<int>& fault_blocks() const { return faults_; }
   const std::vector<int>& hit_blocks() const { return hits_; }
@@ -74,6 +81,10 @@ class FeedbackExtractor {
   std::vector<int> faults_, hits_, dup_hits_;
   uint32_t* fault_bits_ = nullptr;
   uint32_t* hit_bits_ = nullptr;
+  // the second set of bitmaps (allocated, cleared, on first use)
+  uint32_t* fault_bits2_ = nullptr;
+  uint32_t* hit_bits2_ = nullptr;
+  int EnsureSecondSet();
   void* state_ = nullptr;
   void* host_state_ = nullptr;
   // Pinned head of the unique list, read with the state (kPrefixInts codes).

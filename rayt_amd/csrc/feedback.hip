@@ -559,6 +559,8 @@ FeedbackExtractor::~FeedbackExtractor() {
   (void)hipStreamSynchronize(ctx_->stream);
   (void)hipFree(fault_bits_);
   (void)hipFree(hit_bits_);
+  if (fault_bits2_) (void)hipFree(fault_bits2_);
+  if (hit_bits2_) (void)hipFree(hit_bits2_);
   (void)hipFree(state_);
   (void)hipFree(unique_);
   (void)hipFree(temp_);
@@ -604,12 +606,23 @@ int FeedbackExtractor::Process(int* codes, int nodes_in_block, bool block_level)
   return ProcessN(codes, count_, nodes_in_block, block_level);
 }
 
-MarkTargets FeedbackExtractor::Targets(uint32_t fault_limit, uint32_t hit_limit) {
+int FeedbackExtractor::EnsureSecondSet() {
+  if (fault_bits2_) return RT_OK;
+  RT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&fault_bits2_), kFaultDomain / 8));
+  RT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&hit_bits2_), kHitDomain / 8));
+  RT_HIP_CHECK(hipMemsetAsync(fault_bits2_, 0, kFaultDomain / 8, ctx_->stream));
+  RT_HIP_CHECK(hipMemsetAsync(hit_bits2_, 0, kHitDomain / 8, ctx_->stream));
+  RT_HIP_CHECK(hipStreamSynchronize(ctx_->stream));
+  return RT_OK;
+}
+
+MarkTargets FeedbackExtractor::Targets(uint32_t fault_limit, uint32_t hit_limit, int set) {
   premark_fault_limit_ = std::min(fault_limit, kFaultDomain);
   premark_hit_limit_ = std::max(2u, std::min(hit_limit, kHitDomain));
+  if (set && EnsureSecondSet() != RT_OK) throw std::runtime_error(rt_last_error());
   MarkTargets t;
-  t.fault_bits = fault_bits_;
-  t.hit_bits = hit_bits_;
+  t.fault_bits = set ? fault_bits2_ : fault_bits_;
+  t.hit_bits = set ? hit_bits2_ : hit_bits_;
   t.fault_limit = premark_fault_limit_;
   t.hit_limit = premark_hit_limit_;
   return t;
@@ -620,21 +633,27 @@ int FeedbackExtractor::ProcessN(int* codes, int n, int nodes_in_block, bool bloc
   return rc ? rc : CompleteN();
 }
 
-int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked) {
+int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool block_level, bool premarked, int set,
+                                hipStream_t walk_stream, hipEvent_t after) {
   RT_REQUIRE(n >= 0 && n <= count_, RT_ERR_RANGE, "more codes than the extractor was created for");
-  hipStream_t s = stream_;
+  RT_REQUIRE(set == 0 || (set == 1 && premarked && fault_bits2_), RT_ERR_INVALID,
+             "the second bitmap set holds premarked codes only (mark into Targets(..., 1) first)");
+  hipStream_t s = walk_stream ? walk_stream : stream_;
+  uint32_t* fault_bits = set ? fault_bits2_ : fault_bits_;
+  uint32_t* hit_bits = set ? hit_bits2_ : hit_bits_;
   q_codes_ = codes;
   q_n_ = n;
   q_nib_ = nodes_in_block;
   q_block_level_ = block_level;
   q_empty_ = n <= 0;
   if (q_empty_) return RT_OK;
+  if (after) RT_HIP_CHECK(hipStreamWaitEvent(s, after, 0));
   const int bs = block_level ? 0 : nodes_in_block;
   MarkState* st = static_cast<MarkState*>(state_);
   // Premarked: the renderer marked these block-level codes into Targets() as it wrote them.
   const int grid = premarked ? 1 : std::min(Grid(n, kMarkBlock * kMarkItems), max_mark_blocks_);
   if (!premarked)
-    MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, n, bs, fault_bits_, hit_bits_,
+    MarkCodesKernel<<<dim3(grid), dim3(kMarkBlock), 0, s>>>(codes, n, bs, fault_bits, hit_bits,
                                                      static_cast<int4*>(partials_));
   const int2 words = premarked ? make_int2(int((premark_fault_limit_ + 31) / 32), int((premark_hit_limit_ + 31) / 32))
                                : make_int2(0, 0);
@@ -644,12 +663,12 @@ int FeedbackExtractor::EnqueueN(int* codes, int n, int nodes_in_block, bool bloc
   // A premarked walk over a few thousand words takes one 256-thread workgroup;
   // larger walks and the reduction of many partials take 1024 threads.
   if (premarked && words.x + words.y <= 4096)
-    WalkBitmapsKernel<256><<<dim3(1), dim3(256), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+    WalkBitmapsKernel<256><<<dim3(1), dim3(256), 0, s>>>(fault_bits, hit_bits, st, unique_,
                                                          static_cast<const int4*>(partials_), 0, words,
                                                          static_cast<MarkState*>(host_state_),
                                                          static_cast<int*>(host_prefix_), prefix_len_);
   else
-    WalkBitmapsKernel<1024><<<dim3(1), dim3(1024), 0, s>>>(fault_bits_, hit_bits_, st, unique_,
+    WalkBitmapsKernel<1024><<<dim3(1), dim3(1024), 0, s>>>(fault_bits, hit_bits, st, unique_,
                                                            static_cast<const int4*>(partials_), premarked ? 0 : grid,
                                                            words, static_cast<MarkState*>(host_state_),
                                                            static_cast<int*>(host_prefix_), prefix_len_);

@@ -55,6 +55,12 @@ struct rt_tracer {
   int pending_buf = 0;
   std::chrono::steady_clock::time_point pending_t0;
   hipEvent_t pev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // trace start / end per code buffer
+  // The feedback walk of a pipelined frame runs on walk_stream, reading the
+  // bitmap set of the frame's code buffer, while the next frame's trace marks
+  // the other set; walk_done[buf] follows the walk of set buf.
+  hipStream_t walk_stream = nullptr;
+  hipEvent_t walk_done[2] = {nullptr, nullptr};
+  bool walk_queued[2] = {false, false};
   hipEvent_t Event(size_t i) {
     while (ev.size() <= i) {
       hipEvent_t e;
@@ -184,8 +190,14 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
   rt_buf* codes = t->codes_buf[buf];
   rayt::FeedbackExtractor& fx = *t->feedback->fx;
   const rayt::MarkTargets targets =
-      fx.Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
+      fx.Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u, buf);
   const rayt::TileSchedule sched = Schedule(t);
+  if (!t->walk_stream) {
+    RT_HIP_CHECK(hipStreamCreateWithFlags(&t->walk_stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : t->walk_done) RT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  // the walk that last read (and cleared) this bitmap set has finished before the trace marks it
+  if (t->walk_queued[buf]) RT_HIP_CHECK(hipStreamWaitEvent(t->ctx->stream, t->walk_done[buf], 0));
   RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 0), t->ctx->stream));
   int rc = rayt::RenderPlain(t->ctx, t->out_image, codes, t->width, t->height, rt_cache_channel(c, 0),
                              rt_cache_far_pointers(c), m.root_node_index(), rt_cache_channel_by_name(c, "normals"),
@@ -201,8 +213,11 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
   t->codes = codes;
   rc = m.StartRequestSession();
   if (rc) return rc;
-  rc = fx.EnqueueN(static_cast<int*>(codes->ptr), fx.count(), m.nodes_in_block(), true, true);
+  rc = fx.EnqueueN(static_cast<int*>(codes->ptr), fx.count(), m.nodes_in_block(), true, true, buf, t->walk_stream,
+                   PipelineEvent(t, buf, 1));
   if (rc) return rc;
+  RT_HIP_CHECK(hipEventRecord(t->walk_done[buf], t->walk_stream));
+  t->walk_queued[buf] = true;
   t->pending = true;
   t->pending_buf = buf;
   t->pending_t0 = t0;
@@ -260,6 +275,7 @@ int rt_tracer_create(rt_cache* cache, int width, int height, rt_tracer** out) {
 int rt_tracer_destroy(rt_tracer* t) {
   if (!t) return RT_OK;
   (void)rayt::FlushPending(t);
+  if (t->walk_stream) (void)hipStreamSynchronize(t->walk_stream);
   if (t->cache && t->cache->pending == t) t->cache->pending = nullptr;
   rt_buf_destroy(t->out_image);
   rt_buf_destroy(t->codes_buf[0]);
@@ -272,6 +288,9 @@ int rt_tracer_destroy(rt_tracer* t) {
   for (auto& pair : t->pev)
     for (hipEvent_t e : pair)
       if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : t->walk_done)
+    if (e) (void)hipEventDestroy(e);
+  if (t->walk_stream) (void)hipStreamDestroy(t->walk_stream);
   rt_ctx* ctx = t->ctx;
   delete t;
   rayt::CtxRelease(ctx);
