@@ -21,6 +21,8 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out);
 int BatchSetScenes(rt_batch* b, const std::vector<rt_cache*>& caches, const std::vector<int>& depths);
 // Launches of `b` go to stream `s` (nullptr: back to the context stream).
 void BatchSetStream(rt_batch* b, hipStream_t s);
+// Whether launches record the start event rt_batch_last_ms reads (default on).
+void BatchSetTiming(rt_batch* b, bool timing);
 // Enqueues one launch without waiting; a launch with `count` leaves
 // {primary rays, shadow rays, node fetches, hits} for BatchStats. Waits for
 // the previous launch of `b` first (its descriptors are staged in place).

@@ -64,6 +64,13 @@ struct Vec4B {
   float v[4];
 };
 
+// Layout of a batch's descriptor block (see rt_batch).
+constexpr uint64_t kStatsOffset = sizeof(DevScene) * rayt::kMaxBatchScenes;
+constexpr uint64_t kTilePrefixOffset = kStatsOffset + 8ull * 4 * kStatSlots;
+static_assert(kStatsOffset % 16 == 0 && kTilePrefixOffset % 16 == 0, "descriptor block alignment");
+inline uint64_t ViewsOffset(int nviews) { return (kTilePrefixOffset + 4ull * (nviews + 1) + 15) & ~uint64_t(15); }
+inline uint64_t DescBytes(int nviews) { return ViewsOffset(nviews) + sizeof(DevView) * uint64_t(nviews); }
+
 // One wave tile of the sequence: the view holding it (a binary search over
 // the tile prefix, on wave-uniform values, so scalar loads), then one pixel per
 // lane through RenderPixel.
@@ -261,16 +268,15 @@ struct rt_batch {
   int tree_depth = 0;   // depth assumed for scenes registered without one (0: unknown)
   std::vector<rt_cache*> scenes;
   std::vector<int> depths;  // per scene; 0 = tree_depth
-  DevScene* d_scenes = nullptr;
-  DevView* d_views = nullptr;
-  uint32_t* d_tile_begin = nullptr;
+  // The launch's descriptors in one block, copied to the device in one transfer:
+  // [scenes (kMaxBatchScenes) | zeroed statistics (kStatSlots x 4) | tile prefix (nviews + 1) | views (nviews)];
+  // the pinned host copy is rewritten only after the previous launch has completed.
+  uint8_t* d_block = nullptr;
+  uint8_t* h_block = nullptr;
   uint32_t* d_counters = nullptr;
-  unsigned long long* d_stats = nullptr;
-  // Pinned descriptor staging, rewritten only after the previous launch has completed.
-  DevView* h_views = nullptr;
-  uint32_t* h_tile_begin = nullptr;
-  DevScene* h_scenes = nullptr;
   unsigned long long* h_stats = nullptr;
+  bool timing = true;  // record the start event rt_batch_last_ms reads
+  bool last_timed = false;
   bool launched = false;
   bool counted = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -285,15 +291,12 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
   b->ctx = CtxRetain(ctx);
   b->stream = ctx->stream;
   b->max_views = max_views;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(DevView) * max_views);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_tile_begin), 4 * (max_views + 1));
+  const uint64_t block = DescBytes(max_views);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&b->d_block), block);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_counters), 4 * 8 * kCounterStride);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_stats), 8 * 4 * kStatSlots);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->d_scenes), sizeof(DevScene) * kMaxBatchScenes);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_views), sizeof(DevView) * max_views, 0);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_tile_begin), 4 * (max_views + 1), 0);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_scenes), sizeof(DevScene) * kMaxBatchScenes, 0);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_block), block, 0);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stats), 8 * 4 * kStatSlots, 0);
+  if (e == hipSuccess) memset(b->h_block, 0, block);  // the counters' area stays zero
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
   if (e != hipSuccess) {
@@ -305,6 +308,8 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
 }
 
 void BatchSetStream(rt_batch* b, hipStream_t s) { b->stream = s ? s : b->ctx->stream; }
+
+void BatchSetTiming(rt_batch* b, bool timing) { b->timing = timing; }
 
 hipEvent_t BatchDoneEvent(rt_batch* b) { return b->ev1; }
 
@@ -337,7 +342,7 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
     RT_REQUIRE(v.pixel_offset <= out_px && px <= out_px - v.pixel_offset && v.pixel_offset <= codes_px &&
                    px <= codes_px - v.pixel_offset,
                RT_ERR_RANGE, "view outside the output buffers");
-    DevView& d = b->h_views[i];
+    DevView& d = reinterpret_cast<DevView*>(b->h_block + ViewsOffset(nviews))[i];
     memcpy(d.m, v.view_proj_inv, sizeof(d.m));
     d.width = v.width;
     d.height = v.height;
@@ -346,11 +351,11 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
     d.tiles_x = (v.width + tw - 1) / tw;
     d.scene = v.scene;
     d.pixel_offset = v.pixel_offset;
-    b->h_tile_begin[i] = uint32_t(tiles);
+    reinterpret_cast<uint32_t*>(b->h_block + kTilePrefixOffset)[i] = uint32_t(tiles);
     tiles += uint64_t(d.tiles_x) * ((v.height + th - 1) / th);
     RT_REQUIRE(tiles < (1ull << 31), RT_ERR_RANGE, "too many tiles in one batch");
   }
-  b->h_tile_begin[nviews] = uint32_t(tiles);
+  reinterpret_cast<uint32_t*>(b->h_block + kTilePrefixOffset)[nviews] = uint32_t(tiles);
   // The window-only stack needs every scene's declared depth to fit the LDS window.
   bool spill = false;
   for (size_t i = 0; i < b->scenes.size(); ++i) {
@@ -359,21 +364,18 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
     RT_REQUIRE(c->mgr->root_node_index() >= 0, RT_ERR_STATE, "scene has no root block in its pool");
     const int depth = b->depths[i] > 0 ? b->depths[i] : b->tree_depth;
     spill = spill || depth <= 0 || depth - 1 > b->lds_levels;
-    b->h_scenes[i] = SceneOf(c, depth, b->lds_levels);
+    reinterpret_cast<DevScene*>(b->h_block)[i] = SceneOf(c, depth, b->lds_levels);
   }
   hipStream_t s = b->stream;
-  RT_HIP_CHECK(hipMemcpyAsync(b->d_views, b->h_views, sizeof(DevView) * nviews, hipMemcpyHostToDevice, s));
-  RT_HIP_CHECK(hipMemcpyAsync(b->d_tile_begin, b->h_tile_begin, 4 * (nviews + 1), hipMemcpyHostToDevice, s));
-  RT_HIP_CHECK(hipMemcpyAsync(b->d_scenes, b->h_scenes, sizeof(DevScene) * b->scenes.size(), hipMemcpyHostToDevice,
-                              s));
-  ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8 * kCounterStride);
-  if (count) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<uint32_t*>(b->d_stats), 8 * kStatSlots);
+  // one transfer: the scenes, the zeroed statistics, the tile prefix and the views
+  RT_HIP_CHECK(hipMemcpyAsync(b->d_block, b->h_block, DescBytes(nviews), hipMemcpyHostToDevice, s));
+  if (b->wgs_per_cu != 0) ZeroCountersKernel<<<dim3(1), dim3(64), 0, s>>>(b->d_counters, 8 * kCounterStride);
   LaunchArgs a;
   a.stream = s;
-  a.views = b->d_views;
+  a.views = reinterpret_cast<const DevView*>(b->d_block + ViewsOffset(nviews));
   a.nviews = nviews;
-  a.scenes = b->d_scenes;
-  a.tile_begin = b->d_tile_begin;
+  a.scenes = reinterpret_cast<const DevScene*>(b->d_block);
+  a.tile_begin = reinterpret_cast<const uint32_t*>(b->d_block + kTilePrefixOffset);
   a.total_tiles = uint32_t(tiles);
   a.counters = b->d_counters;
   a.out = static_cast<uint32_t*>(out_image->ptr);
@@ -389,13 +391,15 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   const uint64_t waves = (std::max<uint64_t>(want, 1) + 7) & ~uint64_t(7);
   a.grid = dim3(uint32_t(waves));
   a.grab = uint32_t(std::clamp<uint64_t>(tiles / (waves * 16), 1, kMaxTilesPerGrab));
-  a.stats = count ? b->d_stats : nullptr;
+  a.stats = count ? reinterpret_cast<unsigned long long*>(b->d_block + kStatsOffset) : nullptr;
   a.lod = any_lod;
-  RT_HIP_CHECK(hipEventRecord(b->ev0, s));
+  b->last_timed = b->timing;
+  if (b->timing) RT_HIP_CHECK(hipEventRecord(b->ev0, s));
   const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");
   RT_HIP_CHECK(hipGetLastError());
-  if (count) RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_stats, 8 * 4 * kStatSlots, hipMemcpyDeviceToHost, s));
+  if (count)
+    RT_HIP_CHECK(hipMemcpyAsync(b->h_stats, b->d_block + kStatsOffset, 8 * 4 * kStatSlots, hipMemcpyDeviceToHost, s));
   RT_HIP_CHECK(hipEventRecord(b->ev1, s));
   b->launched = true;
   b->counted = count;
@@ -430,14 +434,9 @@ int rt_batch_create(rt_cache* cache, int max_views, rt_batch** out) {
 int rt_batch_destroy(rt_batch* b) {
   if (!b) return RT_OK;
   if (b->launched) (void)hipEventSynchronize(b->ev1);
-  (void)hipFree(b->d_views);
-  (void)hipFree(b->d_tile_begin);
+  (void)hipFree(b->d_block);
   (void)hipFree(b->d_counters);
-  (void)hipFree(b->d_stats);
-  (void)hipFree(b->d_scenes);
-  if (b->h_views) (void)hipHostFree(b->h_views);
-  if (b->h_tile_begin) (void)hipHostFree(b->h_tile_begin);
-  if (b->h_scenes) (void)hipHostFree(b->h_scenes);
+  if (b->h_block) (void)hipHostFree(b->h_block);
   if (b->h_stats) (void)hipHostFree(b->h_stats);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -481,6 +480,7 @@ int rt_batch_render(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_i
 int rt_batch_last_ms(rt_batch* b, double* ms) {
   RT_REQUIRE(b && ms, RT_ERR_INVALID, "null argument");
   RT_REQUIRE(b->launched, RT_ERR_STATE, "nothing launched yet");
+  RT_REQUIRE(b->last_timed, RT_ERR_STATE, "the last launch was not timed");
   RT_HIP_CHECK(hipEventSynchronize(b->ev1));
   float f = 0;
   RT_HIP_CHECK(hipEventElapsedTime(&f, b->ev0, b->ev1));

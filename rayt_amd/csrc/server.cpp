@@ -289,6 +289,7 @@ int rt_server::Launch(Slot& s) {
   int rc = rayt::BatchSetScenes(s.batch, caches, depths);
   if (rc) return rc;
   rayt::BatchSetStream(s.batch, s.stream);
+  rayt::BatchSetTiming(s.batch, false);
   for (int id : s.scenes)  // the scene's last block loads, queued on the context stream
     if (scenes[id].load_queued) RT_HIP_CHECK(hipStreamWaitEvent(s.stream, scenes[id].loaded, 0));
   rc = rayt::BatchLaunch(s.batch, views.data(), int(views.size()), arena_image, arena_codes, true);
@@ -502,10 +503,14 @@ void rt_server::Loop() {
       st.max_rounds_in_flight = std::max(st.max_rounds_in_flight, busy);
     }
     if (!progressed) {
+      if (busy > 0) {
+        // Rounds in flight: poll their events again without sleeping (a timed
+        // wait rounds up to the kernel's timer slack, tens of microseconds).
+        std::this_thread::yield();
+        continue;
+      }
       std::unique_lock<std::mutex> lk(mu);
-      if (busy > 0)
-        cv_work.wait_for(lk, std::chrono::microseconds(20));
-      else if (blocked)
+      if (blocked)
         cv_work.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop || !released.empty(); });
       else
         cv_work.wait_for(lk, std::chrono::milliseconds(2),
