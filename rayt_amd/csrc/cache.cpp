@@ -49,6 +49,8 @@ BlockPool::BlockPool(rt_ctx* ctx, const StoredOctreeHeader& header, int max_bloc
     stage_bytes_ = uint64_t(staging_blocks) * content_ + uint64_t(staging_blocks) * 8 + lw_cap * 12 + fw_cap * 8 + 256;
     Check(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_bytes_, hipHostMallocDefault), "hipHostMalloc");
     Check(hipMalloc(reinterpret_cast<void**>(&dev_stage_), stage_bytes_), "hipMalloc staging");
+    pending_copies_.reserve(size_t(staging_blocks));
+    chunk_expected_.assign(size_t(staging_blocks + kChunkBlocks - 1) / kChunkBlocks, 0);
   } catch (...) {
     for (rt_buf* b : channels_) rt_buf_destroy(b);
     rt_buf_destroy(far_);
@@ -92,18 +94,14 @@ uint8_t* BlockPool::StageBlock(int slot, uint32_t tree_block) {
   return p;
 }
 
-void BlockPool::StageBlockCopy(int slot, uint32_t tree_block, const uint8_t* src, bool src_stable) {
-  uint8_t* dst = StageBlock(slot, tree_block);
-  if (src_stable)
-    pending_copies_.push_back({dst, src});
-  else
-    memcpy(dst, src, content_);
-}
-
-// Copy threads shared by the flushes of one pool. A job is the list of a
-// batch's pending payload copies; every participant takes the next copy with
-// one atomic, so copies finish roughly in staging order, and counts the
-// copies done per chunk.
+// Copy threads shared by the batches of one pool. A job is the list of a
+// batch's payload copies, open while the batch is staged: StageBlockCopy
+// publishes each copy as it is staged, so the threads copy while the
+// transaction goes on, and the flush closes the list, joins in, and queues
+// each chunk of kChunkBlocks payloads for transfer once its copies are done.
+// Every participant takes the next copy with one atomic, so copies finish
+// roughly in staging order; a thread whose index is past the published ones
+// waits for more, and leaves once the job is closed below its index.
 struct BlockPool::CopyWorkers {
   std::vector<std::thread> threads;
   std::mutex mu;
@@ -111,18 +109,21 @@ struct BlockPool::CopyWorkers {
   uint64_t generation = 0;
   bool stop = false;
   // the current job
-  const std::pair<uint8_t*, const uint8_t*>* items = nullptr;
-  size_t n = 0;
+  const std::pair<uint8_t*, const uint8_t*>* items = nullptr;  // fixed for the job (reserved vector)
   uint64_t content = 0;
   const uint8_t* base = nullptr;  // host staging: an item's chunk is (dst - base) / (content * kChunkBlocks)
   std::vector<std::atomic<int>> chunk_done;
   std::atomic<size_t> next{0};
-  std::atomic<int> active{0};  // workers inside the job
+  std::atomic<size_t> avail{0};     // copies published
+  std::atomic<bool> closed{false};  // avail is final
+  std::atomic<int> active{0};       // workers inside the job
+  bool open = false;                // a job is running (owner thread only)
 
-  explicit CopyWorkers(int count) {
+  CopyWorkers(int count, size_t max_chunks) : chunk_done(max_chunks) {
     for (int i = 0; i < count; ++i) threads.emplace_back([this] { Run(); });
   }
   ~CopyWorkers() {
+    Close(avail.load());  // drains what was published
     {
       std::lock_guard<std::mutex> lk(mu);
       stop = true;
@@ -131,12 +132,39 @@ struct BlockPool::CopyWorkers {
     for (std::thread& t : threads) t.join();
   }
   int ChunkOf(size_t i) const { return int(uint64_t(items[i].first - base) / (content * kChunkBlocks)); }
-  // Copies until the job runs out; returns after its last copy.
-  void Work() {
-    for (size_t i; (i = next.fetch_add(1)) < n;) {
-      memcpy(items[i].first, items[i].second, content);
-      chunk_done[ChunkOf(i)].fetch_add(1, std::memory_order_release);
+  void Begin(const std::pair<uint8_t*, const uint8_t*>* list, uint64_t bytes, const uint8_t* stage, size_t published) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      items = list;
+      content = bytes;
+      base = stage;
+      for (auto& a : chunk_done) a.store(0, std::memory_order_relaxed);
+      next.store(0);
+      avail.store(published, std::memory_order_release);
+      closed.store(false);
+      active.store(int(threads.size()));
+      ++generation;
     }
+    open = true;
+    cv.notify_all();
+  }
+  void Publish(size_t n) { avail.store(n, std::memory_order_release); }
+  void Close(size_t n) {
+    avail.store(n, std::memory_order_release);
+    closed.store(true, std::memory_order_release);
+  }
+  // The index of the next copy to make, or false once the job is closed below it.
+  bool Take(size_t& i) {
+    i = next.fetch_add(1, std::memory_order_relaxed);
+    while (i >= avail.load(std::memory_order_acquire)) {
+      if (closed.load(std::memory_order_acquire) && i >= avail.load(std::memory_order_acquire)) return false;
+      std::this_thread::yield();
+    }
+    return true;
+  }
+  void CopyOne(size_t i) {
+    memcpy(items[i].first, items[i].second, content);
+    chunk_done[size_t(ChunkOf(i))].fetch_add(1, std::memory_order_release);
   }
   void Run() {
     uint64_t seen = 0;
@@ -147,11 +175,46 @@ struct BlockPool::CopyWorkers {
         if (stop) return;
         seen = generation;
       }
-      Work();
+      for (size_t i; Take(i);) CopyOne(i);
       active.fetch_sub(1, std::memory_order_acq_rel);
     }
   }
 };
+
+void BlockPool::StageBlockCopy(int slot, uint32_t tree_block, const uint8_t* src, bool src_stable) {
+  uint8_t* dst = StageBlock(slot, tree_block);
+  if (!src_stable) {
+    memcpy(dst, src, content_);
+    return;
+  }
+  pending_copies_.push_back({dst, src});  // capacity staging_blocks_: the job's list never moves
+  ++chunk_expected_[size_t(staged_slots_.size() - 1) / kChunkBlocks];
+  const size_t n = pending_copies_.size();
+  if (workers_ && workers_->open) {
+    workers_->Publish(n);
+    // the chunks staged in full whose copies are done cross to the device while staging goes on
+    if (IssueChunks(int(staged_slots_.size()) / kChunkBlocks) != RT_OK) throw std::runtime_error(rt_last_error());
+  } else if (n == kStreamAfter) {  // a large batch: the copy threads start on it now
+    if (!workers_) {
+      const int hw = int(std::max(2u, std::thread::hardware_concurrency()));
+      workers_.reset(new CopyWorkers(std::min(7, hw - 1), chunk_expected_.size()));
+    }
+    workers_->Begin(pending_copies_.data(), content_, host_stage_, n);
+  }
+}
+
+// Transfers, in order, chunks [chunks_issued_, upto) whose copies are all done.
+int BlockPool::IssueChunks(int upto) {
+  const uint64_t end = uint64_t(staged_slots_.size()) * content_;
+  while (chunks_issued_ < upto && workers_->chunk_done[size_t(chunks_issued_)].load(std::memory_order_acquire) ==
+                                      chunk_expected_[size_t(chunks_issued_)]) {
+    const uint64_t b = uint64_t(chunks_issued_) * kChunkBlocks * content_;
+    const uint64_t e = std::min<uint64_t>(end, b + uint64_t(kChunkBlocks) * content_);
+    RT_HIP_CHECK(hipMemcpyAsync(dev_stage_ + b, host_stage_ + b, e - b, hipMemcpyHostToDevice, ctx_->stream));
+    ++chunks_issued_;
+  }
+  return RT_OK;
+}
 
 // The payload copies of a batch (a few MB per transaction out of core) and
 // the transfers of the payload area [0, nb * content) to the device staging
@@ -159,13 +222,8 @@ struct BlockPool::CopyWorkers {
 int BlockPool::CopyAndTransferPayloads(int nb) {
   const size_t n = pending_copies_.size();
   const int nchunks = (nb + kChunkBlocks - 1) / kChunkBlocks;
-  auto transfer = [&](int c) -> int {
-    const uint64_t b = uint64_t(c) * kChunkBlocks * content_;
-    const uint64_t e = std::min<uint64_t>(uint64_t(nb) * content_, b + uint64_t(kChunkBlocks) * content_);
-    RT_HIP_CHECK(hipMemcpyAsync(dev_stage_ + b, host_stage_ + b, e - b, hipMemcpyHostToDevice, ctx_->stream));
-    return RT_OK;
-  };
-  if (n < 8) {  // a small batch: copy here, one transfer
+  std::fill(chunk_expected_.begin(), chunk_expected_.begin() + nchunks, 0);
+  if (!workers_ || !workers_->open) {  // a small batch: copy here, one transfer
     for (auto& cp : pending_copies_) memcpy(cp.first, cp.second, content_);
     pending_copies_.clear();
     if (nb == 0) return RT_OK;
@@ -173,43 +231,23 @@ int BlockPool::CopyAndTransferPayloads(int nb) {
                                 ctx_->stream));
     return RT_OK;
   }
-  if (!workers_) {
-    const int hw = int(std::max(2u, std::thread::hardware_concurrency()));
-    workers_.reset(new CopyWorkers(std::min(7, hw - 1)));
-  }
   CopyWorkers& w = *workers_;
-  std::vector<int> expected(size_t(nchunks), 0);
-  {
-    std::lock_guard<std::mutex> lk(w.mu);
-    w.items = pending_copies_.data();
-    w.n = n;
-    w.content = content_;
-    w.base = host_stage_;
-    std::vector<std::atomic<int>> counts(static_cast<size_t>(nchunks));
-    for (auto& a : counts) a.store(0, std::memory_order_relaxed);
-    w.chunk_done.swap(counts);
-    for (size_t i = 0; i < n; ++i) ++expected[size_t(w.ChunkOf(i))];
-    w.next.store(0);
-    w.active.store(int(w.threads.size()));
-    ++w.generation;
-  }
-  w.cv.notify_all();
-  int issued = 0, rc = RT_OK;
-  auto issue_ready = [&] {
-    while (rc == RT_OK && issued < nchunks &&
-           w.chunk_done[size_t(issued)].load(std::memory_order_acquire) == expected[size_t(issued)])
-      rc = transfer(issued++);
-  };
-  for (size_t i; (i = w.next.fetch_add(1)) < n;) {
-    memcpy(w.items[i].first, w.items[i].second, content_);
-    w.chunk_done[size_t(w.ChunkOf(i))].fetch_add(1, std::memory_order_release);
-    issue_ready();
+  w.Close(n);
+  for (size_t i = 0; i < n; ++i) ++chunk_expected_[size_t(w.ChunkOf(i))];  // recounted: cleared above
+  int rc = RT_OK;
+  for (size_t i; w.Take(i);) {
+    w.CopyOne(i);
+    if (rc == RT_OK) rc = IssueChunks(nchunks);
   }
   while (w.active.load(std::memory_order_acquire) != 0) {  // the workers' last copies
-    issue_ready();
+    if (rc == RT_OK) rc = IssueChunks(nchunks);
     std::this_thread::yield();
   }
-  issue_ready();
+  if (rc == RT_OK) rc = IssueChunks(nchunks);
+  const int issued = chunks_issued_;
+  w.open = false;
+  chunks_issued_ = 0;
+  std::fill(chunk_expected_.begin(), chunk_expected_.begin() + nchunks, 0);
   pending_copies_.clear();
   return rc != RT_OK ? rc : (issued == nchunks ? RT_OK : Fail(RT_ERR_STATE, "payload chunks not transferred"));
 }

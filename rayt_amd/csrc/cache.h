@@ -76,14 +76,19 @@ class BlockPool {
   // the number of link writes issued before it (those into the slot are dead).
   std::unordered_map<int32_t, std::pair<size_t, size_t>> staged_at_;
   bool in_flight_ = false;
-  // Persistent copy threads (started by the first large batch): a batch's
-  // payload copies are taken in staging order by them and the flushing thread,
-  // and each chunk of kChunkBlocks payloads is queued for transfer as soon as
-  // it is copied, so the copies overlap the transfer of earlier chunks.
+  // Persistent copy threads (started by the first large batch): once a batch
+  // holds kStreamAfter payload copies they copy each one as it is staged, the
+  // flushing thread joins them, and each chunk of kChunkBlocks payloads is
+  // queued for transfer as soon as it is copied, so the copies overlap the
+  // rest of the transaction and the transfer of earlier chunks.
   static constexpr int kChunkBlocks = 16;
+  static constexpr size_t kStreamAfter = 8;
   struct CopyWorkers;
   std::unique_ptr<CopyWorkers> workers_;
+  std::vector<int> chunk_expected_;  // per chunk of the batch: its copies made by the copy threads
+  int chunks_issued_ = 0;            // chunks of the batch already transferred
   int CopyAndTransferPayloads(int nb);
+  int IssueChunks(int upto);
   int WaitStaging();
 };
 
