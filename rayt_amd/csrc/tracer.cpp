@@ -34,6 +34,8 @@ struct rt_tracer {
   int sched_waves = 0;
   bool order_ready = false;   // tile_order holds an order for sched_waves (written by a queued rebuild)
   uint64_t sched_frames = 0;  // resident frames since the shape's first
+  float order_view[16] = {};  // view of the frame whose costs made the order
+  float last_view[16] = {};   // view of the previous resident frame
   rt_buf* states = nullptr;
   rt_feedback* feedback = nullptr;
   uint64_t stats[5] = {0, 0, 0, 0, 0};
@@ -112,9 +114,11 @@ int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premar
 // every kReorderFrames-th frame; the frames between keep the order.
 constexpr uint64_t kReorderFrames = 8;
 
-// The tile schedule of the next resident frame (its order is valid when the
-// previous resident frame had the same launch shape).
-rayt::TileSchedule Schedule(rt_tracer* t) {
+// The tile schedule of the next resident frame, of view `vpi`. Its order is
+// valid when it was made for the same launch shape from a frame of the same
+// view: the tiles' costs move with the camera, and an order made for another
+// view starts long tiles late (a moving camera renders in raster order).
+rayt::TileSchedule Schedule(rt_tracer* t, const float vpi[16]) {
   rayt::TileSchedule s;
   uint32_t tiles;
   int waves;
@@ -125,7 +129,7 @@ rayt::TileSchedule Schedule(rt_tracer* t) {
     t->order_ready = false;
     t->sched_frames = 0;
   }
-  s.order_valid = t->order_ready;
+  s.order_valid = t->order_ready && memcmp(t->order_view, vpi, sizeof(t->order_view)) == 0;
   t->sched_waves = waves;
   if (tiles > uint32_t(rayt::kMaxScheduledTiles)) {  // raster order, no costs: nothing to order
     s.order_valid = false;
@@ -135,14 +139,22 @@ rayt::TileSchedule Schedule(rt_tracer* t) {
   return s;
 }
 
-// Queues the rebuild of the tile order after a resident frame when it is due.
-int MaybeReorder(rt_tracer* t, const rayt::TileSchedule& sched) {
+// Queues the rebuild of the tile order after a resident frame of view `vpi`
+// when it is due, or at once when the camera has come to rest on a view the
+// order was not made for.
+int MaybeReorder(rt_tracer* t, const rayt::TileSchedule& sched, const float vpi[16]) {
   if (!sched.cost) return RT_OK;
-  const bool due = t->sched_frames % kReorderFrames == 0;
+  const bool rested = memcmp(vpi, t->last_view, sizeof(t->last_view)) == 0 &&
+                      memcmp(vpi, t->order_view, sizeof(t->order_view)) != 0;
+  memcpy(t->last_view, vpi, sizeof(t->last_view));
+  const bool due = t->sched_frames % kReorderFrames == 0 || rested;
   ++t->sched_frames;
   if (!due) return RT_OK;
   const int rc = rayt::OrderTiles(t->ctx, sched, t->width, t->height);
-  if (rc == RT_OK) t->order_ready = true;
+  if (rc == RT_OK) {
+    t->order_ready = true;
+    memcpy(t->order_view, vpi, sizeof(t->order_view));
+  }
   return rc;
 }
 
@@ -191,7 +203,7 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
   rayt::FeedbackExtractor& fx = *t->feedback->fx;
   const rayt::MarkTargets targets =
       fx.Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u, buf);
-  const rayt::TileSchedule sched = Schedule(t);
+  const rayt::TileSchedule sched = Schedule(t, vpi);
   if (!t->walk_stream) {
     RT_HIP_CHECK(hipStreamCreateWithFlags(&t->walk_stream, hipStreamNonBlocking));
     for (hipEvent_t& e : t->walk_done) RT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -205,7 +217,7 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
                              kBackground, t->flags, nullptr, &targets, sched.cost ? &sched : nullptr);
   if (rc) return rc;
   RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 1), t->ctx->stream));
-  if ((rc = MaybeReorder(t, sched))) return rc;
+  if ((rc = MaybeReorder(t, sched, vpi))) return rc;
   if (t->pending && (rc = CompletePipelined(t))) return rc;
   for (uint64_t& x : t->stats) x = 0;
   t->pass_counts.clear();
@@ -371,7 +383,7 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
         t->feedback->fx->Targets(m.source()->header().blocks_count, 2u * uint32_t(m.cache().max_blocks_count()) + 2u);
     rayt::TileSchedule sched;
     if (resident) {
-      sched = Schedule(t);
+      sched = Schedule(t, vpi);
       rc = rayt::RenderPlain(t->ctx, t->out_image, t->codes, t->width, t->height, links, far_ptrs,
                              m.root_node_index(), normals, colors, nib, vpi, t->lod_voxel_size, kBackground,
                              t->flags, nullptr, premarked ? &targets : nullptr, sched.cost ? &sched : nullptr);
@@ -381,7 +393,7 @@ int rt_tracer_render_frame(rt_tracer* t, const float vpi[16]) {
                               normals, colors, nib, t->flags);
     if (rc) return rc;
     RT_HIP_CHECK(hipEventRecord(t->Event(2 * size_t(iterations) + 1), t->ctx->stream));
-    if (resident && (rc = MaybeReorder(t, sched))) return rc;
+    if (resident && (rc = MaybeReorder(t, sched, vpi))) return rc;
     if (iterations > 0) {
       const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if ((t->max_passes > 0 && iterations + 1 >= t->max_passes) || el > t->frame_time_limit) break;
