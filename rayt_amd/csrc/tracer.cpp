@@ -158,9 +158,15 @@ int MaybeReorder(rt_tracer* t, const rayt::TileSchedule& sched, const float vpi[
   return rc;
 }
 
+// The trace's start and end events time the trace, and the end event orders
+// the feedback walk (another stream of the same device) after it. Neither
+// makes the trace's writes visible to the host, so they are recorded with a
+// device-scope release instead of the default system-scope fence, whose
+// cache writeback put about 11 us between the traces of consecutive frames.
 hipEvent_t PipelineEvent(rt_tracer* t, int buf, int which) {
   hipEvent_t& e = t->pev[buf][which];
-  if (!e && hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+  if (!e && hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess)
+    throw std::runtime_error("hipEventCreate failed");
   return e;
 }
 
@@ -208,8 +214,16 @@ int RenderPipelined(rt_tracer* t, const float vpi[16]) {
     RT_HIP_CHECK(hipStreamCreateWithFlags(&t->walk_stream, hipStreamNonBlocking));
     for (hipEvent_t& e : t->walk_done) RT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  // the walk that last read (and cleared) this bitmap set has finished before the trace marks it
-  if (t->walk_queued[buf]) RT_HIP_CHECK(hipStreamWaitEvent(t->ctx->stream, t->walk_done[buf], 0));
+  // the walk that last read (and cleared) this bitmap set has finished before the trace marks it (a walk
+  // already done needs no wait on the stream, which would put a barrier between the traces)
+  if (t->walk_queued[buf]) {
+    const hipError_t q = hipEventQuery(t->walk_done[buf]);
+    if (q == hipErrorNotReady)
+      RT_HIP_CHECK(hipStreamWaitEvent(t->ctx->stream, t->walk_done[buf], 0));
+    else
+      RT_HIP_CHECK(q);
+    t->walk_queued[buf] = false;
+  }
   RT_HIP_CHECK(hipEventRecord(PipelineEvent(t, buf, 0), t->ctx->stream));
   int rc = rayt::RenderPlain(t->ctx, t->out_image, codes, t->width, t->height, rt_cache_channel(c, 0),
                              rt_cache_far_pointers(c), m.root_node_index(), rt_cache_channel_by_name(c, "normals"),
