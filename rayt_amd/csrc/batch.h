@@ -23,6 +23,9 @@ int BatchSetScenes(rt_batch* b, const std::vector<rt_cache*>& caches, const std:
 void BatchSetStream(rt_batch* b, hipStream_t s);
 // Whether launches record the start event rt_batch_last_ms reads (default on).
 void BatchSetTiming(rt_batch* b, bool timing);
+// Counting launches count the rays and hits but not the node fetches (which
+// then read 0), so no traversal step does counting work (default off).
+void BatchSetRayCounts(rt_batch* b, bool rays_only);
 // Enqueues one launch without waiting; a launch with `count` leaves
 // {primary rays, shadow rays, node fetches, hits} for BatchStats. Waits for
 // the previous launch of `b` first (its descriptors are staged in place).

@@ -71,10 +71,14 @@ static_assert(kStatsOffset % 16 == 0 && kTilePrefixOffset % 16 == 0, "descriptor
 inline uint64_t ViewsOffset(int nviews) { return (kTilePrefixOffset + 4ull * (nviews + 1) + 15) & ~uint64_t(15); }
 inline uint64_t DescBytes(int nviews) { return ViewsOffset(nviews) + sizeof(DevView) * uint64_t(nviews); }
 
+// What a launch counts: nothing (the timing variant), rays, hits and node
+// fetches, or rays and hits only (no per-step work: the server's rounds).
+constexpr int kNoCount = 0, kCountAll = 1, kCountRays = 2;
+
 // One wave tile of the sequence: the view holding it (a binary search over
 // the tile prefix, on wave-uniform values, so scalar loads), then one pixel per
 // lane through RenderPixel.
-template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
+template <int TW, int LDS_LEVELS, int COUNT, bool SPILL, bool LOD>
 __device__ __forceinline__ void RenderTile(uint32_t tile, uint32_t lane, const DevView* __restrict__ views,
                                            int nviews, const DevScene* __restrict__ scenes,
                                            const uint32_t* __restrict__ tile_begin, uint32_t* __restrict__ out,
@@ -104,9 +108,9 @@ __device__ __forceinline__ void RenderTile(uint32_t tile, uint32_t lane, const D
     stack.set_base(sc.lds_base);
     uint32_t color;
     int code;
-    RenderPixel<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT, LOD>(pool, sc.root, sc.normals, sc.colors,
-                                                               sc.nodes_in_block, v.m, x, y, v.width, v.height, v.lod,
-                                                               bg.v, v.flags, lt, stack, color, code, cnt);
+    RenderPixel<SplitStack<LDS_LEVELS, 64, SPILL>, COUNT != kNoCount, LOD, COUNT == kCountAll>(
+        pool, sc.root, sc.normals, sc.colors, sc.nodes_in_block, v.m, x, y, v.width, v.height, v.lod, bg.v, v.flags,
+        lt, stack, color, code, cnt);
     const uint64_t pix = v.pixel_offset + uint64_t(y) * uint32_t(v.width) + uint32_t(x);
     out[pix] = color;
     if (codes) codes[pix] = code;
@@ -120,7 +124,7 @@ __device__ __forceinline__ void RenderTile(uint32_t tile, uint32_t lane, const D
 // 64-view mix).
 // LOD = false: every view of the launch has lod == 0 (no LOD stop test, which
 // shadow rays never need either).
-template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
+template <int TW, int LDS_LEVELS, int COUNT, bool SPILL, bool LOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void BatchKernel(
     const DevView* __restrict__ views, int nviews, const DevScene* __restrict__ scenes,
     const uint32_t* __restrict__ tile_begin, uint32_t total_tiles, uint32_t grab, uint32_t* __restrict__ counters,
@@ -145,13 +149,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void Ba
       RenderTile<TW, LDS_LEVELS, COUNT, SPILL, LOD>(q + 8u * k, lane, views, nviews, scenes, tile_begin, out, codes,
                                                     bg, lt, lds, cnt);
   }
-  if (COUNT) FlushCounters(cnt, stats);
+  if (COUNT != kNoCount) FlushCounters(cnt, stats);
 }
 
 // One wave per tile, dispatched by the hardware (wgs_per_cu == 0): workgroup
 // g renders tile g and exits; there is no queue and no tail of persistent
 // waves.
-template <int TW, int LDS_LEVELS, bool COUNT, bool SPILL, bool LOD>
+template <int TW, int LDS_LEVELS, int COUNT, bool SPILL, bool LOD>
 __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict__ views, int nviews,
                                                       const DevScene* __restrict__ scenes,
                                                       const uint32_t* __restrict__ tile_begin,
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(64) void BatchTileKernel(const DevView* __restrict_
   PixelCounters cnt;
   RenderTile<TW, LDS_LEVELS, COUNT, SPILL, LOD>(blockIdx.x, threadIdx.x, views, nviews, scenes, tile_begin, out,
                                                 codes, bg, lt, lds, cnt);
-  if (COUNT) FlushCounters(cnt, stats);
+  if (COUNT != kNoCount) FlushCounters(cnt, stats);
 }
 
 __global__ void ZeroCountersKernel(uint32_t* c, int n) {
@@ -187,22 +191,22 @@ struct LaunchArgs {
   bool per_tile;   // one wave per tile (BatchTileKernel), grid = total_tiles
 };
 
-// The timing variant comes with and without the LOD stop; the counting variant keeps it.
-template <int TW, int L, bool C, bool SP, bool LD>
+// The timing and ray-counting variants come with and without the LOD stop; the variant that counts fetches keeps it.
+template <int TW, int L, int C, bool SP, bool LD>
 void LaunchTiles(const LaunchArgs& a) {
   BatchTileKernel<TW, L, C, SP, LD><<<dim3(a.total_tiles), dim3(64), 0, a.stream>>>(
       a.views, a.nviews, a.scenes, a.tile_begin, a.out, a.codes, a.bg, a.lt, a.stats);
 }
 
-template <int TW, int L, bool C, bool SP>
+template <int TW, int L, int C, bool SP>
 void Launch(const LaunchArgs& a) {
   if (a.per_tile) {
-    if constexpr (!C) {
+    if constexpr (C != kCountAll) {
       if (!a.lod) return LaunchTiles<TW, L, C, SP, false>(a);
     }
     return LaunchTiles<TW, L, C, SP, true>(a);
   }
-  if constexpr (!C) {
+  if constexpr (C != kCountAll) {
     if (!a.lod) {
       BatchKernel<TW, L, C, SP, false><<<a.grid, dim3(64), 0, a.stream>>>(
           a.views, a.nviews, a.scenes, a.tile_begin, a.total_tiles, a.grab, a.counters, a.out, a.codes, a.bg, a.lt,
@@ -216,7 +220,7 @@ void Launch(const LaunchArgs& a) {
 }
 
 // spill = false selects the window-only stack (every scene fits the LDS window).
-template <bool C>
+template <int C>
 bool Dispatch(int tw, int levels, bool spill, const LaunchArgs& a) {
 #define RT_B(TW, L)                   \
   if (tw == TW && levels == L) {      \
@@ -276,6 +280,7 @@ struct rt_batch {
   uint32_t* d_counters = nullptr;
   unsigned long long* h_stats = nullptr;
   bool timing = true;  // record the start event rt_batch_last_ms reads
+  bool rays_only = false;  // counting launches leave the node fetches out
   bool last_timed = false;
   bool launched = false;
   bool counted = false;
@@ -310,6 +315,8 @@ int BatchCreate(rt_ctx* ctx, int max_views, rt_batch** out) {
 void BatchSetStream(rt_batch* b, hipStream_t s) { b->stream = s ? s : b->ctx->stream; }
 
 void BatchSetTiming(rt_batch* b, bool timing) { b->timing = timing; }
+
+void BatchSetRayCounts(rt_batch* b, bool rays_only) { b->rays_only = rays_only; }
 
 hipEvent_t BatchDoneEvent(rt_batch* b) { return b->ev1; }
 
@@ -395,7 +402,9 @@ int BatchLaunch(rt_batch* b, const rt_view* views, int nviews, rt_buf* out_image
   a.lod = any_lod;
   b->last_timed = b->timing;
   if (b->timing) RT_HIP_CHECK(hipEventRecord(b->ev0, s));
-  const bool ok = count ? Dispatch<true>(tw, b->lds_levels, spill, a) : Dispatch<false>(tw, b->lds_levels, spill, a);
+  const bool ok = !count        ? Dispatch<kNoCount>(tw, b->lds_levels, spill, a)
+                  : b->rays_only ? Dispatch<kCountRays>(tw, b->lds_levels, spill, a)
+                                 : Dispatch<kCountAll>(tw, b->lds_levels, spill, a);
   RT_REQUIRE(ok, RT_ERR_STATE, "no kernel for this batch configuration");
   RT_HIP_CHECK(hipGetLastError());
   if (count)

@@ -23,7 +23,7 @@ __device__ __forceinline__ uint32_t ShadeNode(const uint32_t* normals, const uin
 // Shadow-ray test for a primary hit (extension; no reference counterpart).
 // Cast from just before the hit point (a quarter of the hit voxel back along
 // the primary ray, which is empty space) towards light 1, without LOD.
-template <class Stack, bool COUNT>
+template <class Stack, bool COUNT, bool FETCHES = COUNT>
 __device__ __forceinline__ bool ShadowOccluded(const PoolView& pool, uint32_t root,
                                                float ox, float oy, float oz, float dx, float dy, float dz,
                                                const CastResult& hit, const Lights& lt, Stack& stack,
@@ -33,8 +33,8 @@ __device__ __forceinline__ bool ShadowOccluded(const PoolView& pool, uint32_t ro
   const float tb = cl_max(0.0f, hit.hit_t - eps);
   const float sx = ox + dx * tb, sy = oy + dy * tb, sz = oz + dz * tb;
   CastResult s;
-  CastRay<Stack, COUNT, false>(pool, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
-                               0.0f, 0.0f, stack, s, fetches);
+  CastRay<Stack, FETCHES, false>(pool, root, sx, sy, sz, lt.shadow_dir[0], lt.shadow_dir[1], lt.shadow_dir[2],
+                                 0.0f, 0.0f, stack, s, fetches);
   if (COUNT) ++shadow_rays;
   return s.error_code == 0 && s.fault_block == -1 && s.hit_node != -1;
 }
@@ -52,7 +52,9 @@ struct PixelCounters {
 // (with the optional shadow ray) and FinishTracingFrame. Writes the RGBA8
 // pixel and the feedback code (block-level when nodes_in_block > 0).
 // LOD = false: lod == 0 (the primary ray has no LOD stop).
-template <class Stack, bool COUNT, bool LOD = true>
+// COUNT counts the rays and hits; FETCHES (which needs COUNT) also the node
+// fetches, which costs instructions in every traversal step.
+template <class Stack, bool COUNT, bool LOD = true, bool FETCHES = COUNT>
 __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root, const uint32_t* normals,
                                             const uint32_t* colors, int nodes_in_block, const float* m, int x, int y,
                                             int W, int H, float lod, const float* bg, int flags, const Lights& lt,
@@ -60,8 +62,9 @@ __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root,
   const PrimaryRay r = InitRay(m, x, y, W, H, lod);
   CastResult res;
   uint32_t fetches = 0, shadow_rays = 0;
-  CastRay<Stack, COUNT, LOD>(pool, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
-                             r.direction.z, r.direction.w, r.origin.w, stack, res, fetches);
+  static_assert(COUNT || !FETCHES, "fetches are counted with the rays");
+  CastRay<Stack, FETCHES, LOD>(pool, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y,
+                               r.direction.z, r.direction.w, r.origin.w, stack, res, fetches);
   uint32_t color;
   int code;
   if (res.error_code) {
@@ -81,8 +84,9 @@ __device__ __forceinline__ void RenderPixel(const PoolView& pool, uint32_t root,
       const float nx = float(int8_t(n & 255u)) / 127.f, ny = float(int8_t((n >> 8) & 255u)) / 127.f,
                   nz = float(int8_t((n >> 16) & 255u)) / 127.f;
       if (cl_max(0.f, LightTerm(nx, ny, nz, lt.l1)) > 0.0f)
-        occluded = ShadowOccluded<Stack, COUNT>(pool, root, r.origin.x, r.origin.y, r.origin.z, r.direction.x,
-                                                r.direction.y, r.direction.z, res, lt, stack, fetches, shadow_rays);
+        occluded = ShadowOccluded<Stack, COUNT, FETCHES>(pool, root, r.origin.x, r.origin.y, r.origin.z,
+                                                         r.direction.x, r.direction.y, r.direction.z, res, lt, stack,
+                                                         fetches, shadow_rays);
     }
     color = ShadeNode(normals, colors, res.hit_node, lt, occluded);
     code = HitCode(res.hit_node, res.hit_duplicate, nodes_in_block);

@@ -304,6 +304,7 @@ int rt_server::Launch(Slot& s) {
   if (rc) return rc;
   rayt::BatchSetStream(s.batch, s.stream);
   rayt::BatchSetTiming(s.batch, false);
+  rayt::BatchSetRayCounts(s.batch, true);  // the server reports rays, not node fetches
   for (int id : s.scenes)  // the scene's last block loads, queued on the context stream
     if (scenes[id].load_queued) RT_HIP_CHECK(hipStreamWaitEvent(s.stream, scenes[id].loaded, 0));
   uint64_t round_px = 0;
