@@ -313,12 +313,13 @@ def measure_out_of_core(ctx, tree, depth: int, frac: float, frames: int, W: int,
     oracle is the tests' (tests/test_gpu_frame.py: fault lists, images, pool words and LRU order per frame)."""
     from rayt_amd import api
     info = tree.info()
-    # The stack window of the tracing passes and the fused single-pass frames: the smallest that holds the
-    # whole stack (depth - 1 levels) up to 10 levels, the most that leaves 8 waves per SIMD with 256-lane
-    # workgroups; a deeper tree keeps its deepest 10 levels in LDS and the rarely used top ones in a private
-    # array (depth 14: 421-423 frames/s against 400-401 with the whole 13-level stack in a 14-level window).
+    # The launch shape of the tracing passes and the fused single-pass frames: one wave per workgroup (436-438
+    # frames/s at depth 14 against 420-424 with four), and the smallest stack window that holds the whole
+    # stack (depth - 1 levels) up to 10 levels, the most that leaves 8 waves per SIMD; a deeper tree keeps
+    # its deepest 10 levels in LDS and the rarely used top ones in a private array (with four-wave
+    # workgroups: 421-423 frames/s against 400-401 with the whole 13-level stack in a 14-level window).
     levels = min(l for l in (8, 10, 12, 14, 16, 22) if l >= depth - 1) if depth - 1 <= 10 else 10
-    api.plain_config(ctx, levels, 4, depth)
+    api.plain_config(ctx, levels, 1, depth)
     block_bytes = info.bytes_in_block_content + 36  # channels + far-pointer slots + page-table entry
     if pool_gb > 0:
         max_blocks = max(8, min(info.blocks_count, int(pool_gb * (1 << 30) / block_bytes)))

@@ -129,8 +129,8 @@ int rt_render_plain(rt_ctx* ctx, rt_buf* out_image, rt_buf* codes, int width, in
                     const rt_buf* normals, const rt_buf* colors, int nodes_in_block,
                     const float view_proj_inv[16], float lod_voxel_size, const float background[4], int flags,
                     uint64_t* stats);
-/* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context, whose
- * traversal-stack settings rt_raytracing_pass also uses (always with 256-lane workgroups):
+/* Launch shape of the plain kernel (and of the tracer's resident single-pass frames) on this context, which
+ * rt_raytracing_pass also uses (its workgroups: 16x16 pixels for wg_waves 4, 8x8 for 1):
  * lds_stack_levels (8, 10, 12, 14, 16, 22) traversal-stack levels held in LDS, wg_waves 1 (one wave per
  * workgroup, 8x8 pixels) or 4 (16x16 pixels), tree_depth of the scene (0 = unknown) to keep the deepest
  * levels in LDS. When tree_depth - 1 <= lds_stack_levels the whole stack lives in LDS (no private spill

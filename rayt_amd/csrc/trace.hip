@@ -78,19 +78,25 @@ __global__ __launch_bounds__(256) void InitTracingFrameKernel(float4* __restrict
 }
 
 // -------------------------------------------------------------- RaytracingPass
-// The stack follows the context's launch shape (rt_plain_config): a window of
-// LDS_LEVELS levels from `base`, spilling to a private array (SPILL) or alone.
-template <int LDS_LEVELS, bool SPILL>
-__global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ codes, float4* __restrict__ org,
-                                                            const float4* __restrict__ dir, int4* __restrict__ misc,
-                                                            int W, int H, PoolView pool, uint32_t root,
-                                                            const uint32_t* __restrict__ normals,
-                                                            const uint32_t* __restrict__ colors, int nodes_in_block,
-                                                            int flags, Lights lt, int base) {
-  __shared__ uint2 lds[LDS_LEVELS * 256];
-  const uint32_t tiles_x = (W + 15) / 16;
+// The stack and the workgroup follow the context's launch shape
+// (rt_plain_config): a window of LDS_LEVELS levels from `base`, spilling to a
+// private array (SPILL) or alone; WAVES = 4: 16x16-pixel workgroups of four
+// waves, WAVES = 1: one wave per 8x8 tile.
+template <int WAVES, int LDS_LEVELS, bool SPILL>
+__global__ __launch_bounds__(64 * WAVES) void RaytracingPassKernel(
+    int* __restrict__ codes, float4* __restrict__ org, const float4* __restrict__ dir, int4* __restrict__ misc, int W,
+    int H, PoolView pool, uint32_t root, const uint32_t* __restrict__ normals, const uint32_t* __restrict__ colors,
+    int nodes_in_block, int flags, Lights lt, int base) {
+  constexpr int BLOCK = 64 * WAVES;
+  __shared__ uint2 lds[LDS_LEVELS * BLOCK];
   int x, y;
-  TilePixel(blockIdx.x, tiles_x, x, y);
+  if constexpr (WAVES == 4) {
+    TilePixel(blockIdx.x, (W + 15) / 16, x, y);
+  } else {
+    const uint32_t tiles_x = (W + 7) / 8;
+    x = int((blockIdx.x % tiles_x) * 8u + (threadIdx.x & 7u));
+    y = int((blockIdx.x / tiles_x) * 8u + (threadIdx.x >> 3));
+  }
   if (x >= W || y >= H) return;
   const int64_t i = int64_t(y) * W + x;
   int4 s = misc[i];
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(256) void RaytracingPassKernel(int* __restrict__ co
     codes[i] = kNoHit;
     return;
   }
-  using Stack = SplitStack<LDS_LEVELS, 256, SPILL>;
+  using Stack = SplitStack<LDS_LEVELS, BLOCK, SPILL>;
   StackSpill<SPILL ? LDS_LEVELS : kStackLevels> spill;
   Stack stack;
   stack.spill = spill.v;
@@ -451,23 +457,27 @@ PoolView View(const rt_buf* links, const rt_buf* far_ptrs) {
 }
 
 template <int L, bool SP>
-void LaunchPassL(dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc, int W, int H,
+void LaunchPassL(int waves, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc, int W, int H,
                  const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col,
                  int nib, int flags, const Lights& lt, int base) {
-  RaytracingPassKernel<L, SP><<<g, dim3(256), 0, s>>>(codes, org, dir, misc, W, H, pool, root, nrm, col, nib,
-                                                      flags, lt, base);
+  if (waves == 4)
+    RaytracingPassKernel<4, L, SP><<<dim3(TileGrid(W, H)), dim3(256), 0, s>>>(codes, org, dir, misc, W, H, pool,
+                                                                                root, nrm, col, nib, flags, lt, base);
+  else
+    RaytracingPassKernel<1, L, SP><<<dim3(uint32_t((W + 7) / 8) * uint32_t((H + 7) / 8)), dim3(64), 0, s>>>(
+        codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags, lt, base);
 }
 
-// The pass kernel for the context's stack window (levels, spill, base).
-bool DispatchPass(int levels, bool spill, dim3 g, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc,
+// The pass kernel for the context's launch shape (waves per workgroup) and stack window (levels, spill, base).
+bool DispatchPass(int levels, bool spill, int waves, hipStream_t s, int* codes, float4* org, float4* dir, int4* misc,
                   int W, int H, const PoolView& pool, uint32_t root, const uint32_t* nrm, const uint32_t* col,
                   int nib, int flags, const Lights& lt, int base) {
 #define RT_PASS(L)                                                                                           \
   if (levels == L) {                                                                                         \
     if (spill || L >= kStackLevels)                                                                          \
-      LaunchPassL<L, true>(g, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags, lt, base);   \
+      LaunchPassL<L, true>(waves, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags, lt, base); \
     else                                                                                                     \
-      LaunchPassL<L, L >= kStackLevels>(g, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags,  \
+      LaunchPassL<L, L >= kStackLevels>(waves, s, codes, org, dir, misc, W, H, pool, root, nrm, col, nib, flags, \
                                         lt, base);                                                           \
     return true;                                                                                             \
   }
@@ -556,7 +566,7 @@ int rt_raytracing_pass(rt_ctx* ctx, rt_buf* faults_and_hits, rt_buf* states, int
   if (rc) return rc;
   char* base = static_cast<char*>(states->ptr);
   const Lights lt = rayt::MakeLights();
-  const bool ok = DispatchPass(ctx->plain_lds_levels, ctx->plain_spill, dim3(TileGrid(width, height)), ctx->stream,
+  const bool ok = DispatchPass(ctx->plain_lds_levels, ctx->plain_spill, ctx->plain_waves == 1 ? 1 : 4, ctx->stream,
                                static_cast<int*>(faults_and_hits->ptr), reinterpret_cast<float4*>(base),
                                reinterpret_cast<float4*>(base + 16 * P), reinterpret_cast<int4*>(base + 32 * P),
                                width, height, View(node_links, far_pointers), uint32_t(root_node_index),

@@ -161,7 +161,7 @@ def test_outofcore_depth14_frames_match_oracle(gpu_ctx):
         info = tree.info()
         assert info.blocks_count == 161525
         max_blocks = int(8 * (1 << 30) / (info.bytes_in_block_content + 36))
-        api.plain_config(gpu_ctx, 10, 4, 14)  # as measured: the deepest 10 stack levels in LDS
+        api.plain_config(gpu_ctx, 10, 1, 14)  # as measured: one-wave workgroups, the deepest 10 stack levels in LDS
         m = api.GPUOctreeCacheManager(max_blocks, tree, gpu_ctx)
         m.InitialFillCache()
         t = orc.Tree(path)

@@ -226,13 +226,14 @@ def test_bad_links_end_rays_like_the_oracle(gpu_ctx):
     m.close()
 
 
-# (LDS levels, declared depth) of the pass kernel's stack: spill from level 0 (the default), window-only,
-# spill above a window at level 3, the full LDS stack.
-@pytest.mark.parametrize("levels,depth", [(16, 0), (8, 8), (8, 12), (22, 0)])
+# (waves per workgroup, LDS levels, declared depth) of the pass kernel: spill from level 0 (the default),
+# window-only, spill above a window at level 3, the full LDS stack; one-wave workgroups with a window and
+# with spill above it.
+@pytest.mark.parametrize("waves,levels,depth", [(4, 16, 0), (4, 8, 8), (4, 8, 12), (4, 22, 0), (1, 8, 8), (1, 8, 12)])
 @pytest.mark.parametrize("flags", [0, api.L.RT_FLAG_SHADOWS])
-def test_init_pass_finish_match_oracle(gpu_ctx, levels, depth, flags):
+def test_init_pass_finish_match_oracle(gpu_ctx, waves, levels, depth, flags):
     """InitTracingFrame, one RaytracingPass (node-level codes) and FinishTracingFrame as separate kernels."""
-    api.plain_config(gpu_ctx, levels, 4, depth)
+    api.plain_config(gpu_ctx, levels, waves, depth)
     try:
         _init_pass_finish(gpu_ctx, flags)
     finally:
