@@ -115,6 +115,34 @@ def test_server_alone_equals_mix_equals_plain(gpu_ctx, scenes):
     assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
 
 
+def test_server_large_and_small_rounds_equal_plain(gpu_ctx, scenes):
+    """A round over two 3840x2160 views' pixels renders into device memory and is copied after the kernel; a
+    smaller one has the kernel write its images into the pinned mirror. Both give the plain render."""
+    rng = np.random.default_rng(5)
+    big = []
+    for k in range(3):
+        yaw, pitch = float(rng.uniform(-180, 180)), float(rng.uniform(-40, 10))
+        pos = tuple(float(x) for x in rng.uniform(0, 1, 3))
+        big.append((camera_vpi(yaw, pitch, pos, aspect=3840 / 2160), 3840, 2160, k % len(scenes)))
+    small = _requests(4, len(scenes), seed=6)
+    srv = _server(gpu_ctx, scenes, max_views_per_round=8, rounds_in_flight=2, framebuffer_bytes=512 << 20)
+    try:
+        served = _serve(srv, big + small)  # one round: 24.9 M pixels
+        st = srv.stats()
+        assert st.rounds == 1
+        served_small = _serve(srv, small)  # one round: under 1 M pixels
+    finally:
+        srv.close()
+    for i, req in enumerate(big + small):
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(served[i][0], pi, err_msg=f"request {i} image (large round) vs plain")
+        np.testing.assert_array_equal(served[i][1], pc, err_msg=f"request {i} codes (large round) vs plain")
+    for i, req in enumerate(small):
+        pi, pc = _plain(gpu_ctx, scenes, req)
+        np.testing.assert_array_equal(served_small[i][0], pi, err_msg=f"request {i} image (small round) vs plain")
+        np.testing.assert_array_equal(served_small[i][1], pc, err_msg=f"request {i} codes (small round) vs plain")
+
+
 def test_server_admission_by_framebuffer(gpu_ctx, scenes):
     """An arena that holds two 640x480 outputs still serves a queue of ten; bigger requests are refused."""
     reqs = [(camera_vpi(*POSES[i % len(POSES)], aspect=4 / 3), 640, 480, i % 2) for i in range(10)]
