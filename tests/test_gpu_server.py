@@ -107,11 +107,15 @@ def test_server_alone_equals_mix_equals_plain(gpu_ctx, scenes):
         c.initial_fill()
         pools.append((c.links(), c.far(), c.root(), c.channel_bytes(1, 4).view(np.uint32),
                       c.channel_bytes(2, 4).view(np.uint32), t.nodes_in_block))
+    shadow_rays = 0
     for i, (vpi, w, h, s) in enumerate(reqs):
         r = orc.render_plain(*pools[s], w, h, vpi, flags=SHADOWS)
         np.testing.assert_array_equal(mix[i][0], r["image"].reshape(-1), err_msg=f"request {i} image vs oracle")
         np.testing.assert_array_equal(mix[i][1], r["codes"].reshape(-1), err_msg=f"request {i} codes vs oracle")
+        shadow_rays += r["stats"][1]
+    # the rounds' ray counts (counted without node fetches) are the oracle's
     assert st.primary_rays == st.pixels == sum(w * h for _, w, h, _ in reqs)
+    assert st.shadow_rays == shadow_rays > 0
     assert 0 < st.p50_ms <= st.p99_ms <= st.max_ms
 
 
