@@ -110,9 +110,12 @@ int FeedbackAndTransaction(rt_tracer* t, rayt::BlockCacheManager& m, bool premar
 }
 
 // Resident frames between rebuilds of the tile order: the rebuild (one
-// workgroup, about 30 us) follows the first frame of a launch shape and then
-// every kReorderFrames-th frame; the frames between keep the order.
-constexpr uint64_t kReorderFrames = 8;
+// workgroup, about 30 us on the context stream) follows the first frame of a
+// launch shape, the first frame at a view the camera comes to rest on, and
+// then every kReorderFrames-th frame; the frames between keep the order. A
+// camera at rest gives the same costs frame after frame, so the periodic
+// rebuild only refreshes the order.
+constexpr uint64_t kReorderFrames = 64;
 
 // The tile schedule of the next resident frame, of view `vpi`. Its order is
 // valid when it was made for the same launch shape from a frame of the same
